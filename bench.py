@@ -1,0 +1,178 @@
+"""Benchmark of the fused coupled D3Q19 + D3Q7 step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload C2] [--prec f32]
+
+Default workload: C2, 256x128x128 overlapping sphere pack (porosity 0.4),
+Langmuir LDF, 10000 steps.  Prints ONE JSON line with MLUPS (all cells, solids
+included), the HBM roofline of the fused kernel (algorithmic bytes per cell
+update x cells / measured step time, against 8 TB/s) and the CPU oracle timed
+serially on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per coupled cell update: 19 + 7 populations read and written once,
+# adsorbed load and lagged concentration read and written once, one flag byte
+B_ALG = {"f32": 225, "f64": 449, "f16": 121, "bf16": 121}
+B_FIELD = {"f32": 4, "f64": 8, "f16": 4, "bf16": 4}  # per-cell porosity / solid-fraction read
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(cfg, prec, budget_s):
+    """The CPU oracle (serial restatement of the reference update) on a bounded sample of the workload."""
+    from oracle import ref_cpu as R
+    from tests.lattice import oracle_setup
+
+    oprec = 8 if prec == "f64" else 4
+    sysm = R.System(oracle_setup(cfg, R), oprec)
+    _, _, _, t1 = sysm.run(1)
+    steps = int(max(1, min(200, budget_s / max(t1, 1e-6))))
+    _, _, _, t = sysm.run(steps)
+    sysm.close()
+    model, ncpu = cpu_info()
+    return {
+        "value": round(cfg.n * steps / t / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{cfg.meta['workload']} {cfg.nx}x{cfg.ny}x{cfg.nz}, {'FP64' if oprec == 8 else 'FP32'} serial oracle, "
+                  f"{steps} steps after 1 untimed step ({t:.1f} s), host '{model}' ({ncpu} logical CPUs)",
+    }
+
+
+def traffic_from_profile(workload, prec):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload.lower()}_{prec}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--dims", default=None, help="override box, e.g. 512x512x512")
+    ap.add_argument("--prec", default="f32", choices=["f32", "f64", "f16", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-outlet", action="store_true", help="skip the per-step outlet reduction")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the slab-decomposed multi-GPU path is a design "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from fluidmaxxer_amd import workloads
+    from fluidmaxxer_amd.coupled import CoupledSim
+
+    dims = tuple(int(v) for v in args.dims.split("x")) if args.dims else None
+    spec_steps = workloads.SPECS[args.workload]["steps"]
+    steps = args.steps if args.steps is not None else spec_steps
+    cfg = workloads.build(args.workload, dims=dims, steps=max(steps + args.warmup + 1, spec_steps))
+    n = cfg.n
+    sim = CoupledSim(cfg, prec=args.prec)
+    log = not args.no_outlet
+
+    # warm-up (untimed), then exactly `steps` steps between two events on the launch stream
+    if args.warmup:
+        sim.run(args.warmup, log_outlet=log)
+    acc = sim.outlet_acc(steps) if log else None
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(sim.stream)
+    sim.launch_only(steps, acc=acc)
+    stop.record(sim.stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    ms_step = ms / steps
+    mlups = n * steps / (ms / 1e3) / 1e6
+    fluid = float((cfg.solid == 0).mean()) if cfg.ns_op == "bgk" else float(np.mean(cfg.field < 0.5))
+    b_alg = B_ALG[args.prec] + (B_FIELD[args.prec] if cfg.field is not None else 0)
+    achieved = b_alg * n / (ms_step / 1e3) / 1e9
+    c_out_last = None
+    if log:
+        import ctypes
+
+        from fluidmaxxer_amd import abi
+        mean = torch.empty(steps, dtype=torch.float64, device=sim.device)
+        abi.call("fmx_outlet_mean", acc.data_ptr(), steps, cfg.ny * cfg.nz, mean.data_ptr(), sim.stream.cuda_stream)
+        torch.cuda.synchronize()
+        c_out_last = float(mean[-1].item())
+
+    traffic = traffic_from_profile(args.workload, args.prec) if dims is None else None
+    line = {
+        "metric": "Coupled D3Q19+D3Q7 MLUPS & %HBM roofline; adjoint/forward time ratio",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "fp32", "f64": "fp64", "f16": "fp16-storage/fp32-compute",
+                  "bf16": "bf16-storage/fp32-compute"}[args.prec],
+        "data": "synthetic (seeded sphere-pack geometry, equilibrium start)",
+        "config": {
+            "workload": f"{args.workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} coupled NS-AD, {cfg.ns_op} NS operator, "
+                        f"{cfg.isotherm} LDF, porosity {cfg.meta['porosity']:.4f}, seed {cfg.meta['seed']}",
+            "cells": n,
+            "fluid_fraction": round(fluid, 4),
+            "fluid_mlups": round(mlups * fluid, 1),
+            "storage": args.prec,
+            "outlet_reduction_per_step": log,
+            "c_out_last_step": c_out_last,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_cell_update": b_alg,
+            "kernel": "k_coupled (one launch per step)",
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

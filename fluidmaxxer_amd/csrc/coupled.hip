@@ -1,0 +1,691 @@
+// coupled.hip -- the fused coupled step: D3Q19 NS stream-collide, velocity hand-off,
+// LDF sorption source and D3Q7 AD stream-collide in ONE kernel launch per step.
+//
+// Reference step (main.cpp:319-336, NSLattice.hpp:188-209, ADLattice.hpp:144-162):
+//   NS:  PreCollision BCs (VelocityDirichletNEQ on f) -> collide+stream f -> f_new
+//        -> PostStreaming BCs (PressureDirichlet on f_new) -> swap
+//   coupling: u = moments of the post-BC populations (Macros.cu:20-61) copied to the AD lattice
+//   sorption: S = -dq from the previous step's phi (LinearLDF.cu / LangmuirLDF.cu)
+//   AD:  PreCollision BCs (tanh / series inlet on g) -> collide+stream g -> g_new, phi
+//        -> PostStreaming BCs (ADDirichlet inlet, ADNeumannOutlet) -> swap
+//
+// How the boundary conditions fold into one launch without changing results:
+//   * Pressure outlet (x = nx-1): its populations are feq(rho_out, u) with u the
+//     moments of the post-collision cell at x = nx-2 (PressureDirichlet.cu:46-92).
+//     The thread of cell (nx-2, y, z) holds exactly those populations in
+//     registers, so it writes the outlet cell too; the outlet thread writes no f.
+//   * AD Neumann outlet: g_new(nx-1) = g_new(nx-2) (ADNeumannOutlet.cu:21-30): the
+//     thread of (nx-2, y, z) stores its g twice; the outlet thread stores only phi.
+//     The outlet cell's own AD collision result (and so its velocity) is never
+//     observable, which is why a pressure outlet requires the Neumann outlet.
+//   * NEQ velocity inlet (x = 0): step s+1 starts by replacing f(0,y,z) with
+//     feq(rho_N, u_BC) + f_N - feq(rho_N, u_N), f_N the populations of (1,y,z) after
+//     step s (VelocityDirichletNEQ.cu:49-96).  The thread of (1,y,z) computes that
+//     at the end of step s and stores it into f_new(0,y,z); the x = 0 thread stores
+//     its own post-collision populations into the `inlet_raw` plane instead, which
+//     is what the reference holds in f(0,y,z) between steps (used for the AD
+//     velocity here and for state read-back).  fmx_coupled_prologue applies the
+//     first step's update.
+//   * ADDirichlet inlet (post-stream): the x = 0 thread stores phi_in * w.
+//   * Pre-collision inlet series (ADDirichletTanhPulse pattern): every pull whose
+//     source cell (or keep-local cell) lies on x = 0 uses phi_in(step) * w_q.
+//   * The LDF of step s reads phi written by step s-1's AD collision; phi is kept
+//     in place as state (read, then overwritten with this step's value).
+//   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
+//     into FMX_OUTLET_SLOTS FP64 accumulators per step.
+#include "fmx_internal.hpp"
+
+namespace fmx {
+
+template <class C> struct CoupledParams {
+    int nx, ny, nz, py, pz;
+    int ns_inlet, ns_outlet, ad_inlet, ad_outlet, isotherm, coupling;
+    int gx, gy, gz;          // logical block grid
+    int block_x, block_y;
+    int xcd_remap;
+    i64 n;
+    C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref, g_ref;
+};
+
+template <int P> struct CoupledPtrs {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    const S *f;
+    S *fn;
+    const S *g;
+    S *gn;
+    C *q;
+    C *phi;
+    const std::uint8_t *flags;
+    const C *field;
+    const C *tau_field;
+    C *inlet_raw;
+    const double *inlet_series;
+    double *outlet_acc;
+};
+
+// Logical block id -> (bx, by, bz).  With xcd_remap the hardware's round-robin
+// assignment of consecutive workgroups to the 8 XCDs is inverted, so each XCD
+// sweeps one contiguous slab of the domain and its L2 sees the neighbouring rows
+// its pulls touch.
+__device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, int &bx, int &by, int &bz) {
+    const unsigned total = unsigned(gx) * unsigned(gy) * unsigned(gz);
+    unsigned b = blockIdx.x;
+    if (remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
+    bx = int(b % unsigned(gx));
+    const unsigned r = b / unsigned(gx);
+    by = int(r % unsigned(gy));
+    bz = int(r / unsigned(gy));
+}
+
+template <int P, int OP>
+__global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
+                                                 i64 acc_row) {
+    using C = typename Storage<P>::C;
+    int bx, by, bz;
+    block_coords(prm.xcd_remap, prm.gx, prm.gy, prm.gz, bx, by, bz);
+    const int x = bx * prm.block_x + int(threadIdx.x);
+    const int y = by * prm.block_y + int(threadIdx.y);
+    const int z = bz;
+    if (x >= prm.nx || y >= prm.ny) return;
+    const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
+    const i64 n = prm.n;
+    const i64 cell = cell_index(x, y, z, nx, ny);
+    const std::uint8_t fl = p.flags[cell];
+    const bool solid = (OP == FMX_NS_BGK) && (fl & FMX_FLAG_SOLID);
+
+    // neighbour coordinates: x keeps local out of domain, y/z wrap when periodic
+    const bool xm_ok = x > 0, xp_ok = x < nx - 1;
+    int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
+    bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
+    if (prm.py) {
+        if (!ym_ok) ym += ny;
+        if (!yp_ok) yp -= ny;
+        ym_ok = yp_ok = true;
+    }
+    if (prm.pz) {
+        if (!zm_ok) zm += nz;
+        if (!zp_ok) zp -= nz;
+        zm_ok = zp_ok = true;
+    }
+    const int ys[3] = {ym, y, yp};
+    const int zs[3] = {zm, z, zp};
+    const bool yok[3] = {ym_ok, true, yp_ok};
+    const bool zok[3] = {zm_ok, true, zp_ok};
+    const bool xok[3] = {xm_ok, true, xp_ok};
+
+    // ---------------------------------------------------------------- NS pull
+    C fo[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
+        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
+        const i64 src = ok ? cell_index(x - cx, ys[1 - cy], zs[1 - cz], nx, ny) : cell;
+        bool bb = solid;
+        if (OP == FMX_NS_BGK && q != 0 && !solid && ok) bb = (p.flags[src] & FMX_FLAG_SOLID) != 0;
+        const i64 addr = bb ? i64(Q19::opp[q]) * n + cell : i64(q) * n + src;
+        fo[q] = ld_pop<P>(p.f, addr, w19<C>(q), prm.f_ref);
+    }
+
+    // ----------------------------------------------------- NS collide (in place)
+    {
+        C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const C v = fo[q];
+            rho += v;
+            jx += v * C(Q19::cx[q]);
+            jy += v * C(Q19::cy[q]);
+            jz += v * C(Q19::cz[q]);
+        }
+        C ux, uy, uz;
+        if (OP == FMX_NS_POROUS) {  // PorousBGKCollision.cu:118-136
+            if (!(rho > C(0))) {
+                rho = prm.rho0;
+                jx = jy = jz = C(0);
+            }
+            const C invR = C(1) / rho;
+            ux = jx * invR;
+            uy = jy * invR;
+            uz = jz * invR;
+            C ph = p.field ? p.field[cell] : C(1);
+            ph = ph < C(0) ? C(0) : (ph > C(1) ? C(1) : ph);
+            ux *= ph;
+            uy *= ph;
+            uz *= ph;
+        } else {  // BGKCollision.cu:149-158
+            if (!(rho > C(0))) {
+                rho = prm.rho0;
+                ux = uy = uz = C(0);
+            } else {
+                const C invR = C(1) / rho;
+                ux = jx * invR;
+                uy = jy * invR;
+                uz = jz * invR;
+            }
+        }
+        const C u2 = ux * ux + uy * uy + uz * uz;
+        C ns = C(0);
+        if (OP == FMX_NS_PBB) {
+            ns = p.field ? p.field[cell] : C(0);
+            ns = ns < C(0) ? C(0) : (ns > C(1) ? C(1) : ns);
+        }
+        C fc[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const C eu = C(Q19::cx[q]) * ux + C(Q19::cy[q]) * uy + C(Q19::cz[q]) * uz;
+            const C cu = C(3) * eu;
+            const C feq = w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - C(1.5) * u2);
+            fc[q] = fo[q] + prm.omega * (feq - fo[q]);
+        }
+        if (OP == FMX_NS_PBB) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) fc[q] = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
+        }
+        if (!solid) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) fo[q] = fc[q];
+        }
+    }
+
+    // ------------------------------------- post-collision moments (Macros.cu:44-60)
+    C r2 = C(0), vx = C(0), vy = C(0), vz = C(0);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        r2 += fo[q];
+        vx += fo[q] * C(Q19::cx[q]);
+        vy += fo[q] * C(Q19::cy[q]);
+        vz += fo[q] * C(Q19::cz[q]);
+    }
+    const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
+    const C ux2 = vx * inv2, uy2 = vy * inv2, uz2 = vz * inv2;
+
+    // ------------------------------------------------------------- NS stores
+    const bool inlet_cell = prm.ns_inlet && x == 0;
+    const bool outlet_cell = prm.ns_outlet && x == nx - 1;
+    if (inlet_cell) {
+        const i64 np = i64(ny) * nz, pc = i64(y) + i64(ny) * z;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) p.inlet_raw[q * np + pc] = fo[q];
+    } else if (!outlet_cell) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) st_pop_nt<P>(p.fn, i64(q) * n + cell, fo[q], C(Q19::inv_w[q]), prm.f_ref);
+    }
+    if (prm.ns_inlet && x == 1) {  // next step's NEQ update of (0, y, z)
+        const C u2N = ux2 * ux2 + uy2 * uy2 + uz2 * uz2;
+        const i64 dst = cell - 1;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const C euN = C(Q19::cx[q]) * ux2 + C(Q19::cy[q]) * uy2 + C(Q19::cz[q]) * uz2;
+            const C cuN = C(3) * euN;
+            const C feqN = w19<C>(q) * r2 * (C(1) + cuN + C(0.5) * cuN * cuN - C(1.5) * u2N);
+            const C euB = C(Q19::cx[q]) * prm.uin_x + C(Q19::cy[q]) * prm.uin_y + C(Q19::cz[q]) * prm.uin_z;
+            const C cuB = C(3) * euB;
+            const C feqB = w19<C>(q) * r2 * (C(1) + cuB + C(0.5) * cuB * cuB - C(1.5) * prm.uin2);
+            st_pop<P>(p.fn, i64(q) * n + dst, feqB + (fo[q] - feqN), C(Q19::inv_w[q]), prm.f_ref);
+        }
+    }
+    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
+        const C u2o = ux2 * ux2 + uy2 * uy2 + uz2 * uz2;
+        const i64 dst = cell + 1;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const C eu = C(Q19::cx[q]) * ux2 + C(Q19::cy[q]) * uy2 + C(Q19::cz[q]) * uz2;
+            const C v = w19<C>(q) * prm.rho_out * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * u2o);
+            st_pop<P>(p.fn, i64(q) * n + dst, v, C(Q19::inv_w[q]), prm.f_ref);
+        }
+    }
+
+    // ------------------------------------------------------ sorption source
+    C src = C(0);
+    if (prm.isotherm && (fl & FMX_FLAG_REGION)) {
+        C c = p.phi[cell];
+        if (c < prm.min_conc) c = prm.min_conc;
+        const C ce = c - prm.min_conc;
+        C qeq;
+        if (prm.isotherm == FMX_ISO_LANGMUIR) {
+            const C bc = prm.iso_b * ce;
+            qeq = prm.iso_a * (bc / (C(1) + bc));
+        } else {
+            qeq = prm.iso_a * ce;
+        }
+        const C qi = p.q[cell];
+        const C dq = prm.k_ldf * (qeq - qi);
+        p.q[cell] = qi + dq;
+        src = -dq;
+    }
+
+    // --------------------------------------------------------------- AD pull
+    const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
+    C gi[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
+        const int sx = ok ? x - cx : x;
+        const i64 s = ok ? cell_index(sx, ys[1 - cy], zs[1 - cz], nx, ny) : cell;
+        if (prm.ad_inlet == 2 && sx == 0) gi[q] = phi_pre * w7<C>(q);
+        else gi[q] = ld_pop<P>(p.g, i64(q) * n + s, w7<C>(q), prm.g_ref);
+    }
+    C phi = C(0);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) phi += gi[q];
+    if (phi < prm.min_conc) phi = prm.min_conc;
+    const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
+    const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
+    C go[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const C eu = C(Q7::cx[q]) * ax + C(Q7::cy[q]) * ay + C(Q7::cz[q]) * az;
+        const C geq = w7<C>(q) * phi * (C(1) + C(4) * eu);
+        go[q] = gi[q] + om * (geq - gi[q]) + w7<C>(q) * src;
+    }
+    p.phi[cell] = phi;
+
+    // ------------------------------------------------------------- AD stores
+    const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
+    if (prm.ad_inlet == 1 && x == 0) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) st_pop_nt<P>(p.gn, i64(q) * n + cell, prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), prm.g_ref);
+    } else if (!ad_out_cell) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) st_pop_nt<P>(p.gn, i64(q) * n + cell, go[q], C(Q7::inv_w[q]), prm.g_ref);
+    }
+    if (prm.ad_outlet && x == nx - 2) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) st_pop<P>(p.gn, i64(q) * n + cell + 1, go[q], C(Q7::inv_w[q]), prm.g_ref);
+    }
+
+    // ------------------------------------------------- outlet breakthrough sum
+    if (p.outlet_acc && x == nx - 1) {
+        const int slot = int((i64(y) + i64(ny) * z) % FMX_OUTLET_SLOTS);
+        unsafeAtomicAdd(p.outlet_acc + acc_row * (FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE) + slot * FMX_OUTLET_STRIDE,
+                        double(phi));
+    }
+}
+
+// First step's inlet update (NEQ on f(0,y,z) from f(1,y,z)) and the raw x = 0 plane.
+template <int P>
+__global__ void __launch_bounds__(256) k_coupled_prologue(typename Storage<P>::S *f,
+                                                          typename Storage<P>::C *inlet_raw,
+                                                          CoupledParams<typename Storage<P>::C> prm) {
+    using C = typename Storage<P>::C;
+    const i64 np = i64(prm.ny) * prm.nz;
+    const i64 k = gtid();
+    if (k >= np) return;
+    const i64 n = prm.n;
+    const i64 c0 = k * prm.nx;  // (0, y, z) with k = y + ny*z
+    const i64 c1 = c0 + 1;
+    C f1[19];
+    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        inlet_raw[q * np + k] = ld_pop<P>(f, i64(q) * n + c0, w19<C>(q), prm.f_ref);
+        f1[q] = ld_pop<P>(f, i64(q) * n + c1, w19<C>(q), prm.f_ref);
+        r += f1[q];
+        jx += f1[q] * C(Q19::cx[q]);
+        jy += f1[q] * C(Q19::cy[q]);
+        jz += f1[q] * C(Q19::cz[q]);
+    }
+    if (!prm.ns_inlet) return;
+    const C inv = (r > C(0)) ? (C(1) / r) : C(0);
+    const C ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    const C u2N = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const C euN = C(Q19::cx[q]) * ux + C(Q19::cy[q]) * uy + C(Q19::cz[q]) * uz;
+        const C cuN = C(3) * euN;
+        const C feqN = w19<C>(q) * r * (C(1) + cuN + C(0.5) * cuN * cuN - C(1.5) * u2N);
+        const C euB = C(Q19::cx[q]) * prm.uin_x + C(Q19::cy[q]) * prm.uin_y + C(Q19::cz[q]) * prm.uin_z;
+        const C cuB = C(3) * euB;
+        const C feqB = w19<C>(q) * r * (C(1) + cuB + C(0.5) * cuB * cuB - C(1.5) * prm.uin2);
+        st_pop<P>(f, i64(q) * n + c0, feqB + (f1[q] - feqN), C(Q19::inv_w[q]), prm.f_ref);
+    }
+}
+
+// Reference-view read-back: decode populations; the x = 0 plane comes from inlet_raw.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_coupled_view(const typename Storage<P>::S *pop,
+                                                      const typename Storage<P>::C *inlet_raw,
+                                                      typename Storage<P>::C *out, int nx, i64 np, i64 n,
+                                                      typename Storage<P>::C ref) {
+    using C = typename Storage<P>::C;
+    const i64 c = gtid();
+    if (c >= n) return;
+    const i64 xx = c % nx;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C w = (Q == 19) ? w19<C>(q) : w7<C>(q);
+        C v = ld_pop<P>(pop, i64(q) * n + c, w, ref);
+        if (inlet_raw && xx == 0) v = inlet_raw[q * np + c / nx];
+        out[q * n + c] = v;
+    }
+}
+
+// Encode populations into storage: dst[q*n + c] = src ? src[q*n + c] : w_q * value.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_encode_pop(typename Storage<P>::S *dst, const typename Storage<P>::C *src,
+                                                    i64 n, typename Storage<P>::C value, typename Storage<P>::C ref) {
+    using C = typename Storage<P>::C;
+    const i64 c = gtid();
+    if (c >= n) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C w = (Q == 19) ? w19<C>(q) : w7<C>(q);
+        const int iw = (Q == 19) ? Q19::inv_w[q] : Q7::inv_w[q];
+        const C v = src ? src[q * n + c] : w * value;
+        st_pop<P>(dst, i64(q) * n + c, v, C(iw), ref);
+    }
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) k_copy_or_zero(C *dst, const C *src, i64 n) {
+    const i64 c = gtid();
+    if (c >= n) return;
+    dst[c] = src ? src[c] : C(0);
+}
+
+__global__ void __launch_bounds__(256) k_outlet_mean(const double *__restrict__ acc, i64 nsteps, double inv_count,
+                                                     double *__restrict__ mean) {
+    const i64 k = gtid();
+    if (k >= nsteps) return;
+    double s = 0.0;
+    for (int j = 0; j < FMX_OUTLET_SLOTS; ++j) s += acc[k * FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE + j * FMX_OUTLET_STRIDE];
+    mean[k] = s * inv_count;
+}
+
+// ------------------------------------------------------------- host side
+struct LaunchShape {
+    int bx = 64, by = 4, remap = 1;
+};
+
+static LaunchShape launch_shape() {
+    LaunchShape s;
+    if (const char *e = std::getenv("FMX_BLOCK")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 256 && (a * b) % 64 == 0) {
+            s.bx = a;
+            s.by = b;
+        }
+    }
+    if (const char *e = std::getenv("FMX_XCD_REMAP")) s.remap = std::atoi(e) != 0;
+    return s;
+}
+
+template <class C> static CoupledParams<C> make_params(const fmx_coupled_desc &d) {
+    CoupledParams<C> p{};
+    const LaunchShape ls = launch_shape();
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.nz = d.nz;
+    p.py = d.periodic_y != 0;
+    p.pz = d.periodic_z != 0;
+    p.ns_inlet = d.ns_inlet;
+    p.ns_outlet = d.ns_outlet;
+    p.ad_inlet = d.ad_inlet;
+    p.ad_outlet = d.ad_outlet;
+    p.isotherm = d.isotherm;
+    p.coupling = d.coupling;
+    p.block_x = ls.bx;
+    p.block_y = ls.by;
+    p.gx = (d.nx + ls.bx - 1) / ls.bx;
+    p.gy = (d.ny + ls.by - 1) / ls.by;
+    p.gz = d.nz;
+    p.xcd_remap = ls.remap;
+    p.n = i64(d.nx) * d.ny * d.nz;
+    C tau = C(d.tau);
+    if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204
+    p.omega = C(1) / tau;
+    p.rho0 = C(d.rho0);
+    p.uin_x = C(d.u_in[0]);
+    p.uin_y = C(d.u_in[1]);
+    p.uin_z = C(d.u_in[2]);
+    p.uin2 = p.uin_x * p.uin_x + p.uin_y * p.uin_y + p.uin_z * p.uin_z;
+    p.rho_out = C(d.rho_out);
+    p.phi_in = C(d.phi_in);
+    p.omega_ad = C(1) / C(d.tau_ad);
+    p.min_conc = C(d.min_conc);
+    p.iso_a = C(d.iso_a);
+    p.iso_b = C(d.iso_b);
+    p.k_ldf = C(d.k_ldf);
+    p.f_ref = C(d.f_ref);
+    p.g_ref = C(d.g_ref);
+    return p;
+}
+
+template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int cur) {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    CoupledPtrs<P> p{};
+    p.f = static_cast<const S *>(b.f[cur]);
+    p.fn = static_cast<S *>(b.f[1 - cur]);
+    p.g = static_cast<const S *>(b.g[cur]);
+    p.gn = static_cast<S *>(b.g[1 - cur]);
+    p.q = static_cast<C *>(b.q);
+    p.phi = static_cast<C *>(b.phi);
+    p.flags = b.flags;
+    p.field = static_cast<const C *>(b.field);
+    p.tau_field = static_cast<const C *>(b.tau_field);
+    p.inlet_raw = static_cast<C *>(b.inlet_raw);
+    p.inlet_series = b.inlet_series;
+    p.outlet_acc = b.outlet_acc;
+    return p;
+}
+
+template <int P, int OP>
+static int run_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
+                     hipStream_t s) {
+    using C = typename Storage<P>::C;
+    const CoupledParams<C> prm = make_params<C>(d);
+    const unsigned nblocks = unsigned(prm.gx) * unsigned(prm.gy) * unsigned(prm.gz);
+    const dim3 block(prm.block_x, prm.block_y, 1);
+    for (int k = 0; k < nsteps; ++k) {
+        const CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
+        hipLaunchKernelGGL((k_coupled<P, OP>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
+        const int rc = after_launch("k_coupled");
+        if (rc) return rc;
+    }
+    return FMX_OK;
+}
+
+template <int P>
+static int run_prec(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
+                    hipStream_t s) {
+    switch (d.ns_op) {
+        case FMX_NS_BGK: return run_steps<P, FMX_NS_BGK>(d, b, step0, nsteps, cur, s);
+        case FMX_NS_POROUS: return run_steps<P, FMX_NS_POROUS>(d, b, step0, nsteps, cur, s);
+        case FMX_NS_PBB: return run_steps<P, FMX_NS_PBB>(d, b, step0, nsteps, cur, s);
+        default: return set_error(FMX_ERR_ARG, "fmx_coupled_run: unknown ns_op %d", d.ns_op);
+    }
+}
+
+template <int P>
+static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const CoupledParams<C> prm = make_params<C>(d);
+    const i64 np = i64(d.ny) * d.nz;
+    hipLaunchKernelGGL(k_coupled_prologue<P>, dim3(blocks_for(np)), dim3(256), 0, s, static_cast<S *>(b.f[cur]),
+                       static_cast<C *>(b.inlet_raw), prm);
+    return after_launch("k_coupled_prologue");
+}
+
+template <int P>
+static int load_state(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, const void *f_c, const void *g_c,
+                      const void *q_c, const void *phi_c, bool init, double phi0, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    const unsigned nb = blocks_for(n);
+    if (init || f_c) {
+        hipLaunchKernelGGL((k_encode_pop<P, 19>), dim3(nb), dim3(256), 0, s, static_cast<S *>(b.f[cur]),
+                           static_cast<const C *>(f_c), n, C(d.rho0), C(d.f_ref));
+        if (int rc = after_launch("k_encode_pop")) return rc;
+    }
+    if (init || g_c) {
+        hipLaunchKernelGGL((k_encode_pop<P, 7>), dim3(nb), dim3(256), 0, s, static_cast<S *>(b.g[cur]),
+                           static_cast<const C *>(g_c), n, C(phi0), C(d.g_ref));
+        if (int rc = after_launch("k_encode_pop")) return rc;
+    }
+    if (init || q_c) {
+        hipLaunchKernelGGL(k_copy_or_zero<C>, dim3(nb), dim3(256), 0, s, static_cast<C *>(b.q), static_cast<const C *>(q_c), n);
+        if (int rc = after_launch("k_copy_or_zero")) return rc;
+    }
+    if (init || phi_c) {
+        hipLaunchKernelGGL(k_copy_or_zero<C>, dim3(nb), dim3(256), 0, s, static_cast<C *>(b.phi),
+                           static_cast<const C *>(phi_c), n);
+        if (int rc = after_launch("k_copy_or_zero")) return rc;
+    }
+    if (b.inlet_raw) return prologue<P>(d, b, cur, s);
+    return FMX_OK;
+}
+
+template <int P, int Q>
+static int view(const fmx_coupled_desc &d, const void *pop, const void *inlet_raw, void *out, double ref,
+                hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    hipLaunchKernelGGL((k_coupled_view<P, Q>), dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const S *>(pop),
+                       static_cast<const C *>(inlet_raw), static_cast<C *>(out), d.nx, i64(d.ny) * d.nz, n, C(ref));
+    return after_launch("k_coupled_view");
+}
+
+}  // namespace fmx
+
+using namespace fmx;
+
+extern "C" {
+
+int fmx_coupled_validate(const fmx_coupled_desc *d, const fmx_coupled_bufs *b) {
+    if (!d || !b) return set_error(FMX_ERR_ARG, "fmx_coupled: null descriptor");
+    if (d->nx < 3 || d->ny < 1 || d->nz < 1 || d->nz >= 65536)
+        return set_error(FMX_ERR_ARG, "fmx_coupled: need nx >= 3, ny >= 1, 1 <= nz < 65536 (got %d x %d x %d)", d->nx,
+                         d->ny, d->nz);
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    if (n > (i64(1) << 40)) return set_error(FMX_ERR_ARG, "fmx_coupled: domain too large");
+    if (d->prec < FMX_F32 || d->prec > FMX_BF16) return set_error(FMX_ERR_ARG, "fmx_coupled: bad precision %d", d->prec);
+    if (d->ns_op < FMX_NS_BGK || d->ns_op > FMX_NS_PBB) return set_error(FMX_ERR_ARG, "fmx_coupled: bad ns_op");
+    if (d->ns_inlet < 0 || d->ns_inlet > 1 || d->ns_outlet < 0 || d->ns_outlet > 1 || d->ad_inlet < 0 ||
+        d->ad_inlet > 2 || d->ad_outlet < 0 || d->ad_outlet > 1 || d->isotherm < 0 || d->isotherm > 2)
+        return set_error(FMX_ERR_ARG, "fmx_coupled: bad boundary/isotherm selector");
+    if (d->ns_outlet && !d->ad_outlet)
+        return set_error(FMX_ERR_UNSUPPORTED,
+                         "fmx_coupled: a pressure outlet needs the AD Neumann outlet in the fused step (use the "
+                         "operator API for other combinations)");
+    if (!(d->tau_ad > 0.0) && !b->tau_field) return set_error(FMX_ERR_ARG, "fmx_coupled: tau_ad must be > 0");
+    if (!b->f[0] || !b->f[1] || !b->g[0] || !b->g[1] || !b->q || !b->phi || !b->flags)
+        return set_error(FMX_ERR_ARG, "fmx_coupled: missing state buffer");
+    if (d->ns_inlet && !b->inlet_raw) return set_error(FMX_ERR_ARG, "fmx_coupled: ns_inlet needs inlet_raw");
+    if (d->ad_inlet == 2 && !b->inlet_series) return set_error(FMX_ERR_ARG, "fmx_coupled: ad_inlet 2 needs inlet_series");
+    if (d->ns_op == FMX_NS_POROUS && !b->field)
+        return FMX_OK;  // porosity defaults to 1 (PorousBGKCollision.cu:130)
+    return FMX_OK;
+}
+
+int fmx_coupled_prologue(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_prologue: cur must be 0 or 1");
+    if (!b->inlet_raw) return FMX_OK;
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return prologue<FMX_F32>(*d, *b, cur, s);
+        case FMX_F64: return prologue<FMX_F64>(*d, *b, cur, s);
+        case FMX_F16: return prologue<FMX_F16>(*d, *b, cur, s);
+        default: return prologue<FMX_BF16>(*d, *b, cur, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_coupled_init(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, double phi0, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_init: cur must be 0 or 1");
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return load_state<FMX_F32>(*d, *b, cur, nullptr, nullptr, nullptr, nullptr, true, phi0, s);
+        case FMX_F64: return load_state<FMX_F64>(*d, *b, cur, nullptr, nullptr, nullptr, nullptr, true, phi0, s);
+        case FMX_F16: return load_state<FMX_F16>(*d, *b, cur, nullptr, nullptr, nullptr, nullptr, true, phi0, s);
+        default: return load_state<FMX_BF16>(*d, *b, cur, nullptr, nullptr, nullptr, nullptr, true, phi0, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_coupled_load(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, const void *f_c, const void *g_c,
+                     const void *q_c, const void *phi_c, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_load: cur must be 0 or 1");
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return load_state<FMX_F32>(*d, *b, cur, f_c, g_c, q_c, phi_c, false, 0.0, s);
+        case FMX_F64: return load_state<FMX_F64>(*d, *b, cur, f_c, g_c, q_c, phi_c, false, 0.0, s);
+        case FMX_F16: return load_state<FMX_F16>(*d, *b, cur, f_c, g_c, q_c, phi_c, false, 0.0, s);
+        default: return load_state<FMX_BF16>(*d, *b, cur, f_c, g_c, q_c, phi_c, false, 0.0, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_coupled_run(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps, int cur,
+                    void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_run: cur must be 0 or 1");
+    if (nsteps < 0 || step0 < 0) return set_error(FMX_ERR_ARG, "fmx_coupled_run: bad step range");
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return run_prec<FMX_F32>(*d, *b, step0, nsteps, cur, s);
+        case FMX_F64: return run_prec<FMX_F64>(*d, *b, step0, nsteps, cur, s);
+        case FMX_F16: return run_prec<FMX_F16>(*d, *b, step0, nsteps, cur, s);
+        default: return run_prec<FMX_BF16>(*d, *b, step0, nsteps, cur, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_coupled_view_f(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, void *f_out, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if ((cur != 0 && cur != 1) || !f_out) return set_error(FMX_ERR_ARG, "fmx_coupled_view_f: bad arguments");
+    const void *raw = d->ns_inlet ? b->inlet_raw : nullptr;
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return view<FMX_F32, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
+        case FMX_F64: return view<FMX_F64, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
+        case FMX_F16: return view<FMX_F16, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
+        default: return view<FMX_BF16, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_coupled_view_g(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, void *g_out, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if ((cur != 0 && cur != 1) || !g_out) return set_error(FMX_ERR_ARG, "fmx_coupled_view_g: bad arguments");
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return view<FMX_F32, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
+        case FMX_F64: return view<FMX_F64, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
+        case FMX_F16: return view<FMX_F16, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
+        default: return view<FMX_BF16, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_outlet_mean(const double *acc, int64_t nsteps, int64_t count, double *mean, void *stream) {
+    FMX_TRY_BEGIN
+    if (nsteps == 0) return FMX_OK;
+    if (!acc || !mean || nsteps < 0 || count <= 0) return set_error(FMX_ERR_ARG, "fmx_outlet_mean: bad arguments");
+    hipLaunchKernelGGL(k_outlet_mean, dim3(blocks_for(nsteps)), dim3(256), 0, as_stream(stream), acc, nsteps,
+                       1.0 / double(count), mean);
+    return after_launch("k_outlet_mean");
+    FMX_TRY_END
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// ops_ad.hip -- D3Q7 advection-diffusion operators, LDF sorption sources, layout
+// conversion and FP64 diagnostics reductions.
+#include "fmx_internal.hpp"
+
+namespace fmx {
+
+// --------------------------------------------------------------- AD init
+// k_ad_init_eq_d3q7 (ADBGKCollision.cu:29-38)
+template <class T> __global__ void __launch_bounds__(256) k_ad_init_eq(T *__restrict__ g, i64 n, T phi0) {
+    const i64 c = gtid();
+    if (c >= n) return;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) g[q * n + c] = phi0 * w7<T>(q);
+}
+
+// ------------------------------------------------------ AD collide+stream
+// k_ad_bgk_stream_pull_d3q7_source (ADBGKCollision.cu:41-128)
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_ad_collide(const T *__restrict__ g, T *__restrict__ g_new, T *__restrict__ phi_out, const T *__restrict__ tau,
+                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
+                 const T *__restrict__ S, fmx_grid gr, T phiMin) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= gr.nx || y >= gr.ny) return;
+    const i64 n = i64(gr.nx) * gr.ny * gr.nz;
+    const i64 cell = cell_index(x, y, z, gr.nx, gr.ny);
+    const T omega = T(1) / tau[cell];
+    const T vx = ux[cell], vy = uy[cell], vz = uz[cell];
+    T gi[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        int xs = x - Q7::cx[q], ys = y - Q7::cy[q], zs = z - Q7::cz[q];
+        bool ok = true;
+        if (gr.periodic_x) xs = (xs % gr.nx + gr.nx) % gr.nx; else ok = ok && xs >= 0 && xs < gr.nx;
+        if (gr.periodic_y) ys = (ys % gr.ny + gr.ny) % gr.ny; else ok = ok && ys >= 0 && ys < gr.ny;
+        if (gr.periodic_z) zs = (zs % gr.nz + gr.nz) % gr.nz; else ok = ok && zs >= 0 && zs < gr.nz;
+        gi[q] = ok ? g[q * n + cell_index(xs, ys, zs, gr.nx, gr.ny)] : g[q * n + cell];
+    }
+    T phi = T(0);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) phi += gi[q];
+    if (phi < phiMin) phi = phiMin;
+    const T src = S ? S[cell] : T(0);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const T eu = T(Q7::cx[q]) * vx + T(Q7::cy[q]) * vy + T(Q7::cz[q]) * vz;
+        const T geq = w7<T>(q) * phi * (T(1) + T(4) * eu);
+        g_new[q * n + cell] = gi[q] + omega * (geq - gi[q]) + w7<T>(q) * src;
+    }
+    if (phi_out) phi_out[cell] = phi;
+}
+
+// k_ad_dirichlet (ADDirichlet.cu:11-30), k_ad_dirichlet_tanh_pulse_d3q7 (ADDirichletTanhPulse.cu:66-86)
+template <class T>
+__global__ void __launch_bounds__(256) k_ad_bc_dirichlet(T *__restrict__ g, const i64 *__restrict__ cells, i64 count,
+                                                         i64 n, T phi) {
+    const i64 k = gtid();
+    if (k >= count) return;
+    const i64 c = cells[k];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) g[q * n + c] = phi * w7<T>(q);
+}
+
+// k_ad_neumann (ADNeumannOutlet.cu:5-31)
+template <class T>
+__global__ void __launch_bounds__(256) k_ad_bc_neumann(T *__restrict__ g, const i64 *__restrict__ cells,
+                                                       const i64 *__restrict__ src, i64 count, i64 n) {
+    const i64 k = gtid();
+    if (k >= count) return;
+    const i64 c = cells[k], s = src[k];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) g[q * n + c] = g[q * n + s];
+}
+
+// ------------------------------------------------------------------ LDF
+// k_henry_ldf_source (LinearLDF.cu:13-48) and k_langmuir_ldf_source (LangmuirLDF.cu:7-41)
+template <class T, bool LANGMUIR>
+__global__ void __launch_bounds__(256)
+    k_ldf(i64 n, const std::uint8_t *__restrict__ region, const T *__restrict__ phi, T *__restrict__ S,
+          T *__restrict__ q, T a, T b, T kLDF, T phiMin) {
+    const i64 i = gtid();
+    if (i >= n) return;
+    if (!region[i]) {
+        S[i] = T(0);
+        return;
+    }
+    T c = phi[i];
+    if (c < phiMin) c = phiMin;
+    const T ce = c - phiMin;
+    T qeq;
+    if (LANGMUIR) {
+        const T bc = b * ce;
+        qeq = a * (bc / (T(1) + bc));
+    } else {
+        qeq = a * ce;
+    }
+    const T qi = q[i];
+    const T dq = kLDF * (qeq - qi);
+    q[i] = qi + dq;
+    S[i] = -dq;
+}
+
+// --------------------------------------------------------------- layout
+template <class T>
+__global__ void __launch_bounds__(256) k_pack_aos(const T *__restrict__ soa, T *__restrict__ aos, int Q, i64 n) {
+    const i64 i = gtid();
+    if (i >= n * Q) return;
+    const i64 c = i / Q;
+    const int q = int(i - c * Q);
+    aos[i] = soa[q * n + c];
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_unpack_aos(const T *__restrict__ aos, T *__restrict__ soa, int Q, i64 n) {
+    const i64 i = gtid();
+    if (i >= n * Q) return;
+    const i64 c = i / Q;
+    const int q = int(i - c * Q);
+    soa[q * n + c] = aos[i];
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero(std::uint32_t *__restrict__ p, i64 words) {
+    const i64 stride = i64(gridDim.x) * blockDim.x;
+    for (i64 i = gtid(); i < words; i += stride) p[i] = 0u;
+}
+__global__ void k_fill_zero_tail(unsigned char *__restrict__ p, i64 nbytes) {
+    const i64 i = gtid();
+    if (i < nbytes) p[i] = 0;
+}
+
+// ---------------------------------------------------------- diagnostics
+// Deterministic two-pass FP64 sums.  Pass 1: each block reduces a fixed
+// contiguous chunk to one partial per quantity; pass 2: one block adds the
+// partials in block order.
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_mass_partial(const typename Storage<P>::C *__restrict__ phi, const typename Storage<P>::C *__restrict__ q,
+                   const typename Storage<P>::S *__restrict__ f, i64 n, typename Storage<P>::C fref, i64 chunk,
+                   double *__restrict__ partial) {
+    using C = typename Storage<P>::C;
+    __shared__ double red[3][256];
+    const i64 begin = i64(blockIdx.x) * chunk;
+    const i64 end = begin + chunk < n ? begin + chunk : n;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (i64 c = begin + threadIdx.x; c < end; c += blockDim.x) {
+        if (phi) s0 += double(phi[c]);
+        if (q) s1 += double(q[c]);
+        if (f) {
+            C r = C(0);
+#pragma unroll
+            for (int k = 0; k < 19; ++k) r += ld_pop<P>(f, k * n + c, w19<C>(k), fref);
+            s2 += double(r);
+        }
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partial[blockIdx.x * 3 + k] = red[k][0];
+}
+
+__global__ void __launch_bounds__(256) k_mass_final(const double *__restrict__ partial, int nblocks,
+                                                    double *__restrict__ out) {
+    __shared__ double red[3][256];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 3; ++k) s[k] += partial[b * 3 + k];
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) out[k] = red[k][0];
+}
+
+template <class T> static int ad_collide(const void *g, void *gn, void *phi, const void *tau, const void *ux,
+                                         const void *uy, const void *uz, const void *S, const fmx_grid &gr,
+                                         double minC, hipStream_t s) {
+    const dim3 block(64, 4, 1);
+    const dim3 grid((gr.nx + 63) / 64, (gr.ny + 3) / 4, gr.nz);
+    hipLaunchKernelGGL(k_ad_collide<T>, grid, block, 0, s, static_cast<const T *>(g), static_cast<T *>(gn),
+                       static_cast<T *>(phi), static_cast<const T *>(tau), static_cast<const T *>(ux),
+                       static_cast<const T *>(uy), static_cast<const T *>(uz), static_cast<const T *>(S), gr, T(minC));
+    return after_launch("k_ad_collide");
+}
+
+template <class T, bool L>
+static int ldf(i64 n, const std::uint8_t *region, const void *phi, void *S, void *q, double a, double b, double k,
+               double minC, hipStream_t s) {
+    hipLaunchKernelGGL((k_ldf<T, L>), dim3(blocks_for(n)), dim3(256), 0, s, n, region, static_cast<const T *>(phi),
+                       static_cast<T *>(S), static_cast<T *>(q), T(a), T(b), T(k), T(minC));
+    return after_launch("k_ldf");
+}
+
+template <int P>
+static int reduce_mass(const void *phi, const void *q, const void *f, i64 n, double fref, double *out, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const i64 chunk = 1 << 16;
+    const int nblocks = int((n + chunk - 1) / chunk);
+    double *partial = nullptr;
+    int rc = check_hip(hipMallocAsync(reinterpret_cast<void **>(&partial), sizeof(double) * 3 * nblocks, s),
+                       "hipMallocAsync");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mass_partial<P>, dim3(nblocks), dim3(256), 0, s, static_cast<const C *>(phi),
+                       static_cast<const C *>(q), static_cast<const S *>(f), n, C(fref), chunk, partial);
+    rc = after_launch("k_mass_partial");
+    if (!rc) {
+        hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+        rc = after_launch("k_mass_final");
+    }
+    int rc2 = check_hip(hipFreeAsync(partial, s), "hipFreeAsync");
+    return rc ? rc : rc2;
+}
+
+}  // namespace fmx
+
+using namespace fmx;
+
+extern "C" {
+
+int fmx_ad_init_eq(int prec, void *g, const fmx_grid *grid, double phi0, void *stream) {
+    FMX_TRY_BEGIN
+    if (!g || !grid || !grid_ok(*grid)) return set_error(FMX_ERR_ARG, "fmx_ad_init_eq: bad arguments");
+    const i64 n = grid_cells(*grid);
+    hipStream_t s = as_stream(stream);
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_ad_init_eq<float>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<float *>(g), n,
+                           float(phi0));
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_ad_init_eq<double>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<double *>(g), n, phi0);
+    else return unsupported_prec("fmx_ad_init_eq", prec);
+    return after_launch("k_ad_init_eq");
+    FMX_TRY_END
+}
+
+int fmx_ad_collide_stream(int prec, const void *g, void *g_new, void *phi_out, const void *tau, const void *ux,
+                          const void *uy, const void *uz, const void *S, const fmx_grid *grid, double min_conc,
+                          void *stream) {
+    FMX_TRY_BEGIN
+    if (!g || !g_new || g == g_new || !tau || !ux || !uy || !uz || !grid || !grid_ok(*grid))
+        return set_error(FMX_ERR_ARG, "fmx_ad_collide_stream: bad arguments");
+    if (prec == FMX_F32) return ad_collide<float>(g, g_new, phi_out, tau, ux, uy, uz, S, *grid, min_conc, as_stream(stream));
+    if (prec == FMX_F64) return ad_collide<double>(g, g_new, phi_out, tau, ux, uy, uz, S, *grid, min_conc, as_stream(stream));
+    return unsupported_prec("fmx_ad_collide_stream", prec);
+    FMX_TRY_END
+}
+
+int fmx_ad_bc_dirichlet(int prec, void *g, const int64_t *cells, int64_t count, double phi, int64_t ncells,
+                        void *stream) {
+    FMX_TRY_BEGIN
+    if (count == 0) return FMX_OK;
+    if (!g || !cells || count < 0 || ncells <= 0) return set_error(FMX_ERR_ARG, "fmx_ad_bc_dirichlet: bad arguments");
+    hipStream_t s = as_stream(stream);
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_ad_bc_dirichlet<float>, dim3(blocks_for(count)), dim3(256), 0, s, static_cast<float *>(g),
+                           cells, count, ncells, float(phi));
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_ad_bc_dirichlet<double>, dim3(blocks_for(count)), dim3(256), 0, s, static_cast<double *>(g),
+                           cells, count, ncells, phi);
+    else return unsupported_prec("fmx_ad_bc_dirichlet", prec);
+    return after_launch("k_ad_bc_dirichlet");
+    FMX_TRY_END
+}
+
+int fmx_ad_bc_neumann(int prec, void *g_new, const int64_t *cells, const int64_t *src, int64_t count, int64_t ncells,
+                      void *stream) {
+    FMX_TRY_BEGIN
+    if (count == 0) return FMX_OK;
+    if (!g_new || !cells || !src || count < 0 || ncells <= 0)
+        return set_error(FMX_ERR_ARG, "fmx_ad_bc_neumann: bad arguments");
+    hipStream_t s = as_stream(stream);
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_ad_bc_neumann<float>, dim3(blocks_for(count)), dim3(256), 0, s,
+                           static_cast<float *>(g_new), cells, src, count, ncells);
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_ad_bc_neumann<double>, dim3(blocks_for(count)), dim3(256), 0, s,
+                           static_cast<double *>(g_new), cells, src, count, ncells);
+    else return unsupported_prec("fmx_ad_bc_neumann", prec);
+    return after_launch("k_ad_bc_neumann");
+    FMX_TRY_END
+}
+
+int fmx_ldf_linear(int prec, int64_t n, const std::uint8_t *region, const void *phi, void *S, void *q, double K,
+                   double k_ldf, double min_conc, void *stream) {
+    FMX_TRY_BEGIN
+    if (n == 0) return FMX_OK;
+    if (n < 0 || !region || !phi || !S || !q) return set_error(FMX_ERR_ARG, "fmx_ldf_linear: bad arguments");
+    if (prec == FMX_F32) return ldf<float, false>(n, region, phi, S, q, K, 0.0, k_ldf, min_conc, as_stream(stream));
+    if (prec == FMX_F64) return ldf<double, false>(n, region, phi, S, q, K, 0.0, k_ldf, min_conc, as_stream(stream));
+    return unsupported_prec("fmx_ldf_linear", prec);
+    FMX_TRY_END
+}
+
+int fmx_ldf_langmuir(int prec, int64_t n, const std::uint8_t *region, const void *phi, void *S, void *q, double q_max,
+                     double b, double k_ldf, double min_conc, void *stream) {
+    FMX_TRY_BEGIN
+    if (n == 0) return FMX_OK;
+    if (n < 0 || !region || !phi || !S || !q) return set_error(FMX_ERR_ARG, "fmx_ldf_langmuir: bad arguments");
+    if (prec == FMX_F32) return ldf<float, true>(n, region, phi, S, q, q_max, b, k_ldf, min_conc, as_stream(stream));
+    if (prec == FMX_F64) return ldf<double, true>(n, region, phi, S, q, q_max, b, k_ldf, min_conc, as_stream(stream));
+    return unsupported_prec("fmx_ldf_langmuir", prec);
+    FMX_TRY_END
+}
+
+int fmx_pack_aos(int prec, const void *soa, void *aos, int Q, int64_t n, void *stream) {
+    FMX_TRY_BEGIN
+    if (!soa || !aos || n <= 0 || (Q != 19 && Q != 7)) return set_error(FMX_ERR_ARG, "fmx_pack_aos: bad arguments");
+    hipStream_t s = as_stream(stream);
+    const i64 tot = n * Q;
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_pack_aos<float>, dim3(blocks_for(tot)), dim3(256), 0, s, static_cast<const float *>(soa),
+                           static_cast<float *>(aos), Q, n);
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_pack_aos<double>, dim3(blocks_for(tot)), dim3(256), 0, s, static_cast<const double *>(soa),
+                           static_cast<double *>(aos), Q, n);
+    else return unsupported_prec("fmx_pack_aos", prec);
+    return after_launch("k_pack_aos");
+    FMX_TRY_END
+}
+
+int fmx_unpack_aos(int prec, const void *aos, void *soa, int Q, int64_t n, void *stream) {
+    FMX_TRY_BEGIN
+    if (!soa || !aos || n <= 0 || (Q != 19 && Q != 7)) return set_error(FMX_ERR_ARG, "fmx_unpack_aos: bad arguments");
+    hipStream_t s = as_stream(stream);
+    const i64 tot = n * Q;
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_unpack_aos<float>, dim3(blocks_for(tot)), dim3(256), 0, s, static_cast<const float *>(aos),
+                           static_cast<float *>(soa), Q, n);
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_unpack_aos<double>, dim3(blocks_for(tot)), dim3(256), 0, s,
+                           static_cast<const double *>(aos), static_cast<double *>(soa), Q, n);
+    else return unsupported_prec("fmx_unpack_aos", prec);
+    return after_launch("k_unpack_aos");
+    FMX_TRY_END
+}
+
+int fmx_fill_zero(void *dst, size_t bytes, void *stream) {
+    FMX_TRY_BEGIN
+    if (bytes == 0) return FMX_OK;
+    if (!dst) return set_error(FMX_ERR_ARG, "fmx_fill_zero: null pointer");
+    hipStream_t s = as_stream(stream);
+    const std::uintptr_t addr = reinterpret_cast<std::uintptr_t>(dst);
+    if (addr % 4 == 0 && bytes >= 4) {
+        const i64 words = i64(bytes / 4);
+        const unsigned nb = blocks_for(words) < 65536u ? blocks_for(words) : 65536u;
+        hipLaunchKernelGGL(k_fill_zero, dim3(nb), dim3(256), 0, s, static_cast<std::uint32_t *>(dst), words);
+        int rc = after_launch("k_fill_zero");
+        if (rc) return rc;
+        const i64 tail = i64(bytes % 4);
+        if (tail) {
+            hipLaunchKernelGGL(k_fill_zero_tail, dim3(1), dim3(4), 0, s, static_cast<unsigned char *>(dst) + words * 4, tail);
+            return after_launch("k_fill_zero_tail");
+        }
+        return FMX_OK;
+    }
+    hipLaunchKernelGGL(k_fill_zero_tail, dim3(blocks_for(i64(bytes))), dim3(256), 0, s, static_cast<unsigned char *>(dst),
+                       i64(bytes));
+    return after_launch("k_fill_zero_tail");
+    FMX_TRY_END
+}
+
+int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
+                    void *stream) {
+    FMX_TRY_BEGIN
+    if (n <= 0 || !out) return set_error(FMX_ERR_ARG, "fmx_reduce_mass: bad arguments");
+    hipStream_t s = as_stream(stream);
+    switch (prec) {
+        case FMX_F32: return reduce_mass<FMX_F32>(phi, q, f, n, f_ref, out, s);
+        case FMX_F64: return reduce_mass<FMX_F64>(phi, q, f, n, f_ref, out, s);
+        case FMX_F16: return reduce_mass<FMX_F16>(phi, q, f, n, f_ref, out, s);
+        case FMX_BF16: return reduce_mass<FMX_BF16>(phi, q, f, n, f_ref, out, s);
+        default: return set_error(FMX_ERR_ARG, "fmx_reduce_mass: unknown precision %d", prec);
+    }
+    FMX_TRY_END
+}
+
+}  // extern "C"

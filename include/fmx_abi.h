@@ -1,0 +1,233 @@
+/* fmx_abi.h -- C ABI of libfmx_hip.so, the gfx950 implementation of
+ * FluidMaxxer's coupled D3Q19 Navier-Stokes + D3Q7 advection-diffusion step.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (FMX_OK = 0) and never throws;
+ *     fmx_last_error() returns the thread's last message.
+ *   - Buffers are caller-owned device pointers (fmx_alloc or any HIP
+ *     allocation).  Scalar fields (phi, q, S, tau, porosity, rho, u) are float
+ *     for FMX_F32/FMX_F16/FMX_BF16 and double for FMX_F64.
+ *   - Populations are stored structure-of-arrays: population q of cell c at
+ *     pop[q*N + c], N = nx*ny*nz, c = x + nx*(y + ny*z).  The reference stores
+ *     array-of-structures f[c*Q + q]; fmx_pack_aos / fmx_unpack_aos convert.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *     kernels are asynchronous on it.  fmx_set_sync_check(1) restores the
+ *     reference's synchronise-and-check after every launch
+ *     (CudaHelpers.cu:16-20).
+ *   - Reference citations are paths relative to the reference repository.
+ */
+#ifndef FMX_ABI_H
+#define FMX_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMX_ABI_VERSION 1
+
+/* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
+enum { FMX_F32 = 0, FMX_F64 = 1, FMX_F16 = 2, FMX_BF16 = 3 };
+/* NS collision operators */
+enum { FMX_NS_BGK = 0, FMX_NS_POROUS = 1, FMX_NS_PBB = 2 };
+/* isotherms */
+enum { FMX_ISO_NONE = 0, FMX_ISO_LINEAR = 1, FMX_ISO_LANGMUIR = 2 };
+/* status codes */
+enum { FMX_OK = 0, FMX_ERR_ARG = 1, FMX_ERR_HIP = 2, FMX_ERR_UNSUPPORTED = 3 };
+/* flag byte bits of the coupled step */
+enum { FMX_FLAG_SOLID = 1, FMX_FLAG_REGION = 2 };
+
+typedef struct fmx_grid {
+    int32_t nx, ny, nz;
+    int32_t periodic_x, periodic_y, periodic_z; /* LBMParams::periodic{X,Y,Z}, Params.hpp:21-23 */
+} fmx_grid;
+
+/* ------------------------------------------------------------- runtime */
+int fmx_abi_version(void);
+const char *fmx_last_error(void);
+int fmx_device_count(int *count);
+/* name (NUL-terminated, truncated to name_len), total HBM bytes, CU count, gcnArchName */
+int fmx_device_info(int device, char *name, int name_len, int64_t *total_mem, int *cu_count, char *arch,
+                    int arch_len);
+int fmx_set_device(int device);
+/* replaces cudaMallocBytes / cudaFreeBytes (CudaHelpers.hpp:8-11, CudaHelpers.cu:23-35) */
+int fmx_alloc(void **ptr, size_t bytes);
+int fmx_free(void *ptr);
+/* replace cudaMemcpyHtoD / DtoH / D2D / cudaMemsetBytes (CudaHelpers.hpp:14-23, CudaHelpers.cu:37-59);
+ * asynchronous on `stream`, host buffers must stay valid until the stream is synchronised */
+int fmx_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int fmx_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int fmx_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int fmx_memset(void *dst, int value, size_t bytes, void *stream);
+/* zero-fill with a kernel (safe inside captured graphs) */
+int fmx_fill_zero(void *dst, size_t bytes, void *stream);
+int fmx_stream_create(void **stream);
+int fmx_stream_destroy(void *stream);
+int fmx_stream_sync(void *stream);
+/* cudaCheckThrow (CudaHelpers.cu:16-20): last launch error + device synchronise */
+int fmx_device_sync(void);
+int fmx_set_sync_check(int on);
+int fmx_event_create(void **event);
+int fmx_event_destroy(void *event);
+int fmx_event_record(void *event, void *stream);
+int fmx_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---------------------------------------------------------- layout I/O */
+/* soa[q*n + c] <-> aos[c*Q + q] for Q in {19, 7} (reference layout, parity I/O) */
+int fmx_pack_aos(int prec, const void *soa, void *aos, int Q, int64_t n, void *stream);
+int fmx_unpack_aos(int prec, const void *aos, void *soa, int Q, int64_t n, void *stream);
+
+/* ----------------------------------------------------- NS operators (F32/F64) */
+/* BGKCollision::initEq -> k_init_eq_d3q19_1d (BGKCollision.cu:44-71, 179-195);
+ * PorousBGKCollision::initEq -> k_init_eq_d3q19 (PorousBGKCollision.cu:43-58): f = w * rho0 */
+int fmx_ns_init_eq(int prec, void *f, const fmx_grid *grid, double rho0, void *stream);
+/* NSCollisionOperator::collideStream (NSInterfaces.hpp:9-15), f -> f_new:
+ *   FMX_NS_BGK    k_bgk_stream_pull_d3q19 (BGKCollision.cu:74-177, 197-223); obstacle nullable
+ *   FMX_NS_POROUS k_porous_bgk_stream_pull_d3q19 (PorousBGKCollision.cu:64-155, 175-197); field = porosity
+ *   FMX_NS_PBB    partial bounce-back (Walsh et al. 2009; no reference counterpart); field = solid fraction
+ * tau is clamped to >= 0.5001 as in BGKCollision.cu:203-204 */
+int fmx_ns_collide_stream(int prec, int op, const void *f, void *f_new, const uint8_t *obstacle, const void *field,
+                          const fmx_grid *grid, double tau, double rho0, void *stream);
+/* Lattice::computeMacroscopicDevice -> k_compute_macros_d3q19 (Macros.cu:20-61, 95-119) */
+int fmx_ns_moments(int prec, const void *f, void *rho, void *ux, void *uy, void *uz, const fmx_grid *grid,
+                   void *stream);
+/* VelocityDirichletNEQ::apply, PreCollision on f (VelocityDirichletNEQ.cu:49-115); bnd/nei are the
+ * compacted boundary and inward-neighbour cell lists built on the host (VelocityDirichletNEQ.hpp:26-107) */
+int fmx_ns_bc_velocity_neq(int prec, void *f, const int64_t *bnd, const int64_t *nei, int64_t count, double ux,
+                           double uy, double uz, int64_t ncells, void *stream);
+/* VelocityDirichletEq::apply, PreCollision on f (VelocityDirichlet.cu:54-84, 111-130) */
+int fmx_ns_bc_velocity_eq(int prec, void *f, const int64_t *cells, int64_t count, double rho0, double ux, double uy,
+                          double uz, int64_t ncells, void *stream);
+/* PressureDirichlet::apply, PostStreaming on f_new (PressureDirichlet.cu:26-96, 123-145); nei[k] is the
+ * x-1 neighbour of cells[k] (mask cells with x = 0 are dropped on the host, PressureDirichlet.cu:50) */
+int fmx_ns_bc_pressure(int prec, void *f_new, const int64_t *cells, const int64_t *nei, int64_t count, double rho_out,
+                       int64_t ncells, void *stream);
+/* BounceBack::apply, PostStreaming on f_new (BounceBack.cu:12-53), as a true swap of opposite pairs */
+int fmx_ns_bc_bounceback(int prec, void *f_new, const int64_t *cells, int64_t count, int64_t ncells, void *stream);
+
+/* ----------------------------------------------------- AD operators (F32/F64) */
+/* ADBGKCollision::initEq -> k_ad_init_eq_d3q7 (ADBGKCollision.cu:29-38, 130-144): g = phi0 * w */
+int fmx_ad_init_eq(int prec, void *g, const fmx_grid *grid, double phi0, void *stream);
+/* ADBGKCollision::collideStream -> k_ad_bgk_stream_pull_d3q7_source (ADBGKCollision.cu:41-128, 146-166);
+ * S and phi_out nullable */
+int fmx_ad_collide_stream(int prec, const void *g, void *g_new, void *phi_out, const void *tau, const void *ux,
+                          const void *uy, const void *uz, const void *S, const fmx_grid *grid, double min_conc,
+                          void *stream);
+/* ADDirichlet (post, on g_new: ADDirichlet.cu:11-53) and ADDirichletTanhPulse (pre, on g:
+ * ADDirichletTanhPulse.cu:66-155): g <- phi * w on the listed cells */
+int fmx_ad_bc_dirichlet(int prec, void *g, const int64_t *cells, int64_t count, double phi, int64_t ncells,
+                        void *stream);
+/* ADNeumannOutlet::apply (ADNeumannOutlet.cu:5-53): g_new[cells[k]] <- g_new[src[k]], all 7 directions */
+int fmx_ad_bc_neumann(int prec, void *g_new, const int64_t *cells, const int64_t *src, int64_t count, int64_t ncells,
+                      void *stream);
+
+/* ------------------------------------------------------------- sorption */
+/* launch_henry_ldf_source_d3q7 (LinearLDF.hpp:8-17, LinearLDF.cu:13-75) */
+int fmx_ldf_linear(int prec, int64_t n, const uint8_t *region, const void *phi, void *S, void *q, double K,
+                   double k_ldf, double min_conc, void *stream);
+/* launch_langmuir_ldf_source_d3q7 (LangmuirLDF.hpp:7-14, LangmuirLDF.cu:7-59) */
+int fmx_ldf_langmuir(int prec, int64_t n, const uint8_t *region, const void *phi, void *S, void *q, double q_max,
+                     double b, double k_ldf, double min_conc, void *stream);
+
+/* ------------------------------------------------ fused coupled step */
+/* One coupled step of main.cpp:319-336 (NS.step -> moments/velocity hand-off -> sorption source ->
+ * AD.step) as a single kernel launch per step, for the configuration the reference's main builds:
+ * x is the flow axis with the inlet plane x = 0 and the outlet plane x = nx-1 (buildXPlaneMask,
+ * helpers.hpp:7-14).  Requires nx >= 3. */
+typedef struct fmx_coupled_desc {
+    int32_t nx, ny, nz;
+    int32_t periodic_y, periodic_z; /* x is never periodic */
+    int32_t prec;                   /* population storage, FMX_F32/F64/F16/BF16 */
+    int32_t ns_op;                  /* FMX_NS_* */
+    int32_t ns_inlet;               /* 0 none, 1 VelocityDirichletNEQ at x = 0 */
+    int32_t ns_outlet;              /* 0 none, 1 PressureDirichlet at x = nx-1 (requires ad_outlet = 1) */
+    int32_t ad_inlet;               /* 0 none, 1 ADDirichlet (post-stream) at x = 0,
+                                       2 pre-collision Dirichlet at x = 0 with a per-step value */
+    int32_t ad_outlet;              /* 0 none, 1 ADNeumannOutlet at x = nx-1 */
+    int32_t isotherm;               /* FMX_ISO_* */
+    int32_t coupling;               /* 1: AD advected by the NS velocity (main.cpp:322-326) */
+    double rho0, tau;               /* LBMParams::rho0, tau */
+    double u_in[3];                 /* inlet velocity */
+    double rho_out;                 /* outlet density */
+    double phi_in;                  /* ADDirichlet value */
+    double tau_ad;                  /* AD relaxation time when tau_field is NULL */
+    double min_conc;                /* LBMParams::minConcentration */
+    double iso_a, iso_b, k_ldf;     /* linear: a = KHenry, k = kA; Langmuir: a = qMax, b = bLangmuir, k = kLDF */
+    double f_ref, g_ref;            /* half storage: p = w * (ref + h) */
+} fmx_coupled_desc;
+
+#define FMX_OUTLET_SLOTS 64
+#define FMX_OUTLET_STRIDE 16 /* doubles between slots (one 128-byte line each) */
+
+typedef struct fmx_coupled_bufs {
+    void *f[2];               /* 19*N populations, ping-pong */
+    void *g[2];               /* 7*N populations, ping-pong */
+    void *q;                  /* N adsorbed load (AdsorbedField) */
+    void *phi;                /* N concentration: clamped pre-collision sum, the next step's sorption input */
+    const uint8_t *flags;     /* N: FMX_FLAG_SOLID | FMX_FLAG_REGION */
+    const void *field;        /* N porosity (POROUS) or solid fraction (PBB); NULL = 1 / 0 */
+    const void *tau_field;    /* N AD relaxation times; NULL = tau_ad */
+    void *inlet_raw;          /* 19*ny*nz, required with ns_inlet = 1: the x = 0 plane's populations as the
+                                 reference holds them between steps (before the next inlet update) */
+    const double *inlet_series; /* ad_inlet = 2: inlet value of step s at inlet_series[s] */
+    double *outlet_acc;       /* nullable: FMX_OUTLET_SLOTS*FMX_OUTLET_STRIDE doubles per step, accumulating
+                                 phi over x = nx-1; step s (of a run starting at step0) at
+                                 outlet_acc[(s - step0) * SLOTS * STRIDE]; caller zeroes it */
+} fmx_coupled_bufs;
+
+int fmx_coupled_validate(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs);
+/* Lattice::initEquilibrium + ADLattice::initEquilibrium (NSLattice.hpp:177-185, ADLattice.hpp:134-141,
+ * BGKCollision.cu:44-71, ADBGKCollision.cu:29-38) for the fused state: f[cur] = w*rho0, g[cur] = w*phi0,
+ * q = 0, phi = 0 (AdsorbedField.hpp:19-20, ADLattice.hpp:49), followed by fmx_coupled_prologue */
+int fmx_coupled_init(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, double phi0, void *stream);
+/* Loads a state given in compute precision SoA (f_c 19*N, g_c 7*N, q_c N, phi_c N; each nullable = keep)
+ * into f[cur], g[cur], q, phi, followed by fmx_coupled_prologue */
+int fmx_coupled_load(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, const void *f_c,
+                     const void *g_c, const void *q_c, const void *phi_c, void *stream);
+/* Copies the x = 0 plane of f[cur] to inlet_raw and applies the first step's inlet update to f[cur]
+ * (call once after initialisation, before the first fmx_coupled_run) */
+int fmx_coupled_prologue(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *stream);
+/* Steps step0 .. step0+nsteps-1.  State is read from f[cur], g[cur]; step k writes buffer
+ * (cur + k + 1) % 2, so the result is in (cur + nsteps) % 2. */
+int fmx_coupled_run(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
+                    int cur, void *stream);
+/* The NS populations of f[cur] as the reference holds them (x = 0 plane from inlet_raw), converted to
+ * compute precision (float or double) SoA in f_out (19*N). */
+int fmx_coupled_view_f(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *f_out,
+                       void *stream);
+/* g[cur] converted to compute precision SoA in g_out (7*N). */
+int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *g_out,
+                       void *stream);
+/* OutletCSV::log average (OutletCSV.hpp:51-80) for nsteps steps from outlet_acc: mean[k] = sum / count */
+int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, double *mean, void *stream);
+
+/* ---------------------------------------------------------- diagnostics */
+/* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);
+ * f is SoA in `prec` storage (half storage needs f_ref); any input may be NULL (sum = 0) */
+int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
+                    void *stream);
+
+/* ------------------------------------------------------------ geometry */
+/* Seeded sphere pack (SplitMix64 -> xoshiro256**): solid walls on y in {0, ny-1} and z in {0, nz-1},
+ * random spheres with integer-voxelised radius r ~ U[r_min, r_max], centres kept `clearance` cells away
+ * from x = 0 and x = nx-1, added until the fluid fraction reaches target_porosity or max_tries
+ * consecutive rejections (overlap = 0 forbids sphere overlap).  solid[c] = 1 for walls and spheres,
+ * sphere[c] = 1 for sphere voxels only (nullable).  *porosity = fluid cells / all cells. */
+int fmx_geom_sphere_pack(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double r_min, double r_max,
+                         double target_porosity, int32_t overlap, int32_t clearance, int64_t max_tries,
+                         uint8_t *solid, uint8_t *sphere, double *porosity);
+/* buildXPlaneMask (helpers.hpp:7-14) */
+int fmx_build_x_plane_mask(int32_t nx, int32_t ny, int32_t nz, int32_t x_plane, uint8_t *mask);
+/* ASCII VTK ImageData with one UInt8 DataArray, x-fastest (the layout readVTI expects) */
+int fmx_vti_write_mask(const char *path, int32_t nx, int32_t ny, int32_t nz, const uint8_t *mask, const char *name);
+/* readVTI (VTIReader.hpp:35-160): pass mask = NULL to query dims; mask[c] = value > threshold
+ * (3-component arrays use the magnitude; a short array fills the rest with 0) */
+int fmx_vti_read_mask(const char *path, double threshold, int32_t *dims, uint8_t *mask, int64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMX_ABI_H */

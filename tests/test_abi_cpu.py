@@ -1,0 +1,117 @@
+"""CPU checks of the C ABI library: it loads, exports everything the header
+declares, validates descriptors, and its host-side geometry / VTI code matches
+the reference's own readVTI and buildXPlaneMask (built into oracle/_ref from the
+reference headers; skipped where that build is absent)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from fluidmaxxer_amd import abi, geometry
+from fluidmaxxer_amd.coupled import CoupledConfig
+from oracle import ref_cpu as R
+
+
+def test_library_exports_every_header_symbol():
+    L = abi.lib()
+    names = abi.header_functions()
+    assert len(names) >= 45
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(abi._SIGS), set(names) ^ set(abi._SIGS)
+    assert L.fmx_abi_version() == 1
+
+
+def test_validate_rejects_bad_descriptors():
+    n = 4 * 3 * 3
+    cfg = CoupledConfig(4, 3, 3, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
+    d = cfg.desc(abi.F32)
+    b = abi.CoupledBufs()
+    for i in range(2):
+        b.f[i] = b.g[i] = 0x1000
+    b.q = b.phi = b.flags = b.inlet_raw = 0x1000
+    assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 0
+    d.nx = 2
+    assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) != 0
+    assert b"nx >= 3" in abi.lib().fmx_last_error()
+    d.nx = 4
+    d.ad_outlet = 0
+    assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 3
+    d.ad_outlet = 1
+    b.inlet_raw = None
+    assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 1
+    with pytest.raises(abi.FmxError):
+        abi.call("fmx_coupled_run", C.byref(d), C.byref(b), 1, 1, 0, None)
+
+
+def test_sphere_pack_deterministic_and_on_target():
+    a = geometry.sphere_pack(64, 32, 32, 1, 4, 4, 0.6, overlap=False)
+    b = geometry.sphere_pack(64, 32, 32, 1, 4, 4, 0.6, overlap=False)
+    c = geometry.sphere_pack(64, 32, 32, 2, 4, 4, 0.6, overlap=False)
+    assert np.array_equal(a.solid, b.solid) and not np.array_equal(a.solid, c.solid)
+    assert abs(a.porosity - 0.6) < 0.01
+    assert np.isclose(a.porosity, 1 - a.solid.mean())
+    s = a.solid.reshape(32, 32, 64)
+    assert s[0].all() and s[-1].all() and s[:, 0].all() and s[:, -1].all()   # z and y walls
+    sph = a.sphere.reshape(32, 32, 64)
+    assert not sph[:, :, :2].any() and not sph[:, :, -2:].any()                # inlet/outlet planes stay clear
+    assert (a.solid >= a.sphere).all()
+    d = geometry.sphere_pack(96, 48, 48, 2, 4, 8, 0.4, overlap=True)
+    assert abs(d.porosity - 0.4) < 0.01
+
+
+def test_x_plane_mask_matches_reference():
+    m = geometry.x_plane_mask(7, 5, 3, 6)
+    assert m.sum() == 15 and m.reshape(3, 5, 7)[:, :, 6].all()
+    if not R.ref_vti_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    assert np.array_equal(m, R.ref_build_x_plane_mask(7, 5, 3, 6))
+
+
+def _write(path, text):
+    with open(path, "w") as fh:
+        fh.write(text)
+
+
+def test_vti_reader_matches_reference(tmp_path):
+    if not R.ref_vti_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    p = geometry.sphere_pack(20, 10, 12, 7, 2, 3, 0.5)
+    f1 = str(tmp_path / "scalar.vti")
+    geometry.write_vti(f1, 20, 10, 12, p.solid)
+    cases = [f1]
+    rng = np.random.default_rng(0)
+    vals = rng.random(4 * 3 * 2)
+    body = " ".join(f"{v:.6f}" for v in vals)
+    f2 = str(tmp_path / "float.vti")
+    _write(f2, f'<VTKFile><ImageData WholeExtent="0 3 0 2 0 1"><Piece Extent="0 3 0 2 0 1"><PointData>'
+               f'<DataArray type="Float32" Name="a" format="ascii">{body}</DataArray></PointData></Piece>'
+               f'</ImageData></VTKFile>')
+    cases.append(f2)
+    v3 = rng.random(3 * 24) * 0.6
+    f3 = str(tmp_path / "vec.vti")
+    _write(f3, '<VTKFile><ImageData WholeExtent="1 4 0 2 0 1"><PointData><DataArray NumberOfComponents="3">'
+               + " ".join(f"{v:.5f}" for v in v3) + "</DataArray></PointData></ImageData></VTKFile>")
+    cases.append(f3)
+    f4 = str(tmp_path / "short.vti")
+    _write(f4, '<VTKFile><ImageData><Piece Extent="0 3 0 2 0 1"><DataArray>1 0 1 0.7 0.2 end 1 1'
+               "</DataArray></Piece></ImageData></VTKFile>")
+    cases.append(f4)
+    for path in cases:
+        dims, mine = geometry.read_vti(path, 0.5)
+        rdims, ref = R.ref_read_vti(path, 0.5)
+        assert dims == rdims, path
+        assert np.array_equal(mine, ref), path
+
+
+def test_vti_reader_errors(tmp_path):
+    f = str(tmp_path / "bad.vti")
+    _write(f, "<VTKFile><ImageData></ImageData></VTKFile>")
+    with pytest.raises(abi.FmxError, match="Extent"):
+        geometry.read_vti(f)
+    _write(f, '<VTKFile><ImageData WholeExtent="0 1 0 1 0 1"></ImageData></VTKFile>')
+    with pytest.raises(abi.FmxError, match="DataArray"):
+        geometry.read_vti(f)
+    with pytest.raises(abi.FmxError, match="could not open"):
+        geometry.read_vti(str(tmp_path / "missing.vti"))

@@ -1,0 +1,165 @@
+"""The fused coupled step (one launch per step) against the oracle's restatement
+of the reference step sequence (main.cpp:319-336), on identical seeded inputs.
+
+Compared after N steps: rho, u (moments of the populations as the reference
+holds them), C (phi), adsorbed load q, the AD populations g and the outlet
+breakthrough curve C_out(t).  Error = max|a - b| / max|b| per field.
+Tolerances: FP32 vs the FP32 oracle <= 1e-5; FP64 vs the FP64 oracle <= 1e-11.
+Half-precision storage is compared with the FP64 oracle and its errors printed.
+"""
+import numpy as np
+import pytest
+
+from fluidmaxxer_amd import workloads
+from fluidmaxxer_amd.coupled import CoupledConfig, CoupledSim
+from fluidmaxxer_amd.geometry import sphere_pack
+from oracle import ref_cpu as R
+from tests.gpu_util import host, say
+from tests.lattice import W19, W7, aos_to_soa, oracle_setup, rel_err, soa_to_aos
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": (4, 1e-5), "f64": (8, 1e-11)}
+
+
+def small_config(variant, dims=(24, 12, 10), seed=11):
+    nx, ny, nz = dims
+    pack = sphere_pack(nx, ny, nz, seed, 2.0, 3.0, 0.6, overlap=False, clearance=4)
+    cfg = CoupledConfig(nx, ny, nz, solid=pack.solid, region=pack.sphere, tau=0.8, u_in=(0.05, 0.0, 0.0),
+                        tau_ad=0.7, isotherm="linear", iso_a=0.5, k_ldf=0.05)
+    rng = np.random.default_rng(seed)
+    n = cfg.n
+    if variant == "porous":
+        cfg.ns_op = "porous"
+        cfg.field = np.where(pack.solid == 1, 0.67, 1.0)
+    elif variant == "pbb":
+        cfg.ns_op = "pbb"
+        cfg.field = np.clip(pack.solid + rng.uniform(-0.05, 0.05, n), 0, 1)
+        cfg.solid = np.zeros_like(pack.solid)
+    elif variant == "langmuir_series":
+        cfg.isotherm, cfg.iso_a, cfg.iso_b, cfg.k_ldf = "langmuir", 2.0, 5.0, 0.02
+        cfg.ad_inlet = "series"
+        cfg.inlet_series = workloads.series_per_step(workloads.series_knots(12, seed), 400)
+    elif variant == "uncoupled":
+        cfg.coupling = False
+        cfg.isotherm = None
+    elif variant == "closed_walls":
+        cfg.periodic_y = cfg.periodic_z = False
+        cfg.tau_field = np.where(pack.solid == 1, 0.9, 0.6)
+    elif variant == "no_inlet_outlet":
+        cfg.ns_inlet = cfg.ns_outlet = False
+        cfg.ad_outlet = False
+        cfg.ad_inlet = "none"
+        cfg.phi0 = 0.4
+    return cfg
+
+
+def perturbed_state(cfg, seed, as_float32):
+    """Random +-1% perturbation of the initial equilibria (so every branch is live from step 1)."""
+    rng = np.random.default_rng(seed)
+    n = cfg.n
+    f = np.outer(W19, np.full(n, cfg.rho0)) * (1 + 0.01 * rng.standard_normal((19, n)))
+    g = np.outer(W7, np.full(n, max(cfg.phi0, 0.05))) * (1 + 0.01 * rng.standard_normal((7, n)))
+    if as_float32:
+        f, g = f.astype(np.float32).astype(np.float64), g.astype(np.float32).astype(np.float64)
+    return f, g
+
+
+def run_pair(cfg, prec, steps, perturb=True, seed=5):
+    oprec, _ = TOL.get(prec, (8, None))
+    say(f"coupled prec={prec} dims={(cfg.nx, cfg.ny, cfg.nz)} op={cfg.ns_op} steps={steps}")
+    sim = CoupledSim(cfg, prec=prec)
+    orc = R.System(oracle_setup(cfg, R), oprec)
+    if perturb:
+        f, g = perturbed_state(cfg, seed, prec != "f64")
+        sim.load(f=f, g=g)
+        orc.set(R.F, soa_to_aos(f, 19))
+        orc.set(R.G, soa_to_aos(g, 7))
+    outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
+    c_gpu = host(sim.run(steps))
+    c_ref, _, _, _ = orc.run(steps, outlet)
+    rho, ux, uy, uz = (host(t) for t in sim.moments())
+    got = dict(rho=rho, ux=ux, uy=uy, uz=uz, phi=host(sim.phi), q=host(sim.q), g=host(sim.view_g()).ravel(),
+               c_out=c_gpu)
+    ref = dict(rho=orc.get(R.RHO), ux=orc.get(R.UX), uy=orc.get(R.UY), uz=orc.get(R.UZ), phi=orc.get(R.PHI),
+               q=orc.get(R.Q), g=aos_to_soa(orc.get(R.G), 7).ravel(), c_out=c_ref)
+    errs = {k: rel_err(got[k], ref[k]) for k in got}
+    say("  errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    return errs
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("variant", ["bgk", "porous", "pbb", "langmuir_series", "uncoupled", "closed_walls",
+                                     "no_inlet_outlet"])
+def test_fused_step_matches_oracle(prec, variant):
+    cfg = small_config(variant)
+    errs = run_pair(cfg, prec, 150)
+    tol = TOL[prec][1]
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("dims", [(3, 5, 4), (4, 3, 3), (5, 1, 7)])
+def test_fused_step_small_extents(dims):
+    """Smallest supported boxes: the x = 1 and x = nx-2 threads carry both boundary updates."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = np.random.default_rng(3)
+    cfg = CoupledConfig(nx, ny, nz, solid=(rng.random(n) < 0.2).astype(np.uint8),
+                        region=(rng.random(n) < 0.5).astype(np.uint8), tau=0.9, tau_ad=0.8, k_ldf=0.1)
+    errs = run_pair(cfg, "f64", 40)
+    assert max(errs.values()) < 1e-11, errs
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_c1_full_run(prec):
+    """C1 (64x32x32 sphere pack, linear isotherm), 1000 steps from the equilibrium start."""
+    cfg = workloads.build("C1")
+    errs = run_pair(cfg, prec, 1000, perturb=False)
+    tol = TOL[prec][1]
+    assert all(v <= tol for v in errs.values()), errs
+
+
+def test_c1_fp32_against_fp64_oracle():
+    """FP32 storage and arithmetic against the FP64 oracle: the per-field error is reported."""
+    cfg = workloads.build("C1")
+    say("C1 f32 vs f64 oracle")
+    sim = CoupledSim(cfg, prec="f32")
+    orc = R.System(oracle_setup(cfg, R), 8)
+    outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
+    c = host(sim.run(1000))
+    cr, _, _, _ = orc.run(1000, outlet)
+    rho, ux, _, _ = (host(t) for t in sim.moments())
+    errs = dict(rho=rel_err(rho, orc.get(R.RHO)), ux=rel_err(ux, orc.get(R.UX)), phi=rel_err(host(sim.phi), orc.get(R.PHI)),
+                q=rel_err(host(sim.q), orc.get(R.Q)), c_out=rel_err(c, cr))
+    say("  f32 vs f64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(v < 1e-4 for v in errs.values()), errs
+
+
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_half_storage_against_fp64(prec):
+    """FP16 / BF16 deviation storage with FP32 arithmetic against the FP64 oracle on C1 (errors reported)."""
+    cfg = workloads.build("C1")
+    say(f"C1 {prec} vs f64 oracle")
+    sim = CoupledSim(cfg, prec=prec)
+    orc = R.System(oracle_setup(cfg, R), 8)
+    outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
+    c = host(sim.run(1000))
+    cr, _, _, _ = orc.run(1000, outlet)
+    rho, ux, _, _ = (host(t) for t in sim.moments())
+    errs = dict(rho=rel_err(rho, orc.get(R.RHO)), ux=rel_err(ux, orc.get(R.UX)), phi=rel_err(host(sim.phi), orc.get(R.PHI)),
+                q=rel_err(host(sim.q), orc.get(R.Q)), c_out=rel_err(c, cr))
+    say(f"  {prec} vs f64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["c_out"] < 5e-2, errs
+
+
+def test_mass_reduction_matches_host_sums():
+    cfg = small_config("bgk")
+    say("mass reduction")
+    sim = CoupledSim(cfg, prec="f64")
+    sim.run(30)
+    m = host(sim.mass())
+    f = host(sim.f[sim.cur])
+    assert np.isclose(m[0], host(sim.phi).sum(), rtol=1e-13)
+    assert np.isclose(m[1], host(sim.q).sum(), rtol=1e-13)
+    assert np.isclose(m[2], f.sum(), rtol=1e-13)

@@ -1,0 +1,247 @@
+"""Each gfx950 operator kernel against its oracle restatement, one application on
+seeded random inputs (perturbed equilibria, random masks), FP32 and FP64.
+
+Tolerances: FP32 <= 2e-6 and FP64 <= 1e-13 relative L-infinity error (the
+kernels and the oracle evaluate the same expressions; the difference is FMA
+contraction and division rounding on the GPU)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fluidmaxxer_amd import abi
+from oracle import ref_cpu as R
+from tests.gpu_util import dev, host, say, stream, sync, torch
+from tests.lattice import W19, W7, aos_to_soa, coords, rel_err, soa_to_aos, x_plane
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [(abi.F32, 4, 2e-6), (abi.F64, 8, 1e-13)]
+
+
+def _tdt(prec):
+    t = torch()
+    return t.float32 if prec == abi.F32 else t.float64
+
+
+def _rand_f(n, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    return np.tile(W19, n) * (1 + amp * rng.standard_normal(n * 19))
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+@pytest.mark.parametrize("op", [abi.NS_BGK, abi.NS_POROUS, abi.NS_PBB])
+@pytest.mark.parametrize("periodic", [(False, True, True), (True, False, True), (False, False, False)])
+def test_ns_collide_stream(prec, oprec, tol, op, periodic):
+    dims = (13, 9, 7)
+    n = 13 * 9 * 7
+    say(f"ns_collide op={op} prec={prec} periodic={periodic} dims={dims}")
+    rng = np.random.default_rng(op * 10 + prec)
+    f = _rand_f(n, 1 + op)
+    obst = (rng.random(n) < 0.3).astype(np.uint8)
+    field = rng.uniform(-0.1, 1.1, n)  # exercises both clamps
+    tdt = _tdt(prec)
+    fin = dev(aos_to_soa(f, 19).ravel(), tdt)
+    fout = torch().empty_like(fin)
+    ob = dev(obst, torch().uint8) if op == abi.NS_BGK else None
+    fl = dev(field, tdt) if op != abi.NS_BGK else None
+    g = abi.grid(*dims, periodic)
+    abi.call("fmx_ns_collide_stream", prec, op, fin.data_ptr(), fout.data_ptr(), ob.data_ptr() if ob is not None else None,
+             fl.data_ptr() if fl is not None else None, C.byref(g), 0.62, 1.0, stream())
+    sync()
+    ref = R.op_ns_collide(oprec, {abi.NS_BGK: R.BGK, abi.NS_POROUS: R.POROUS, abi.NS_PBB: R.PBB}[op], dims, periodic, f,
+                          obstacle=obst if op == abi.NS_BGK else None, field=field if op != abi.NS_BGK else None,
+                          tau=0.62, rho0=1.0)
+    assert rel_err(soa_to_aos(host(fout), 19), ref) < tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+def test_ns_collide_density_fallback_and_tau_clamp(prec, oprec, tol):
+    """rho <= 0 takes the rho0 / u = 0 branch; tau below 0.5001 is clamped."""
+    dims = (6, 5, 4)
+    n = 120
+    say(f"ns_collide fallback prec={prec}")
+    f = _rand_f(n, 3)
+    f[: 19 * 10] = -np.abs(f[: 19 * 10])  # negative density in the first cells
+    tdt = _tdt(prec)
+    for op, rop in ((abi.NS_BGK, R.BGK), (abi.NS_POROUS, R.POROUS)):
+        fin = dev(aos_to_soa(f, 19).ravel(), tdt)
+        fout = torch().empty_like(fin)
+        g = abi.grid(*dims, (True, True, True))
+        abi.call("fmx_ns_collide_stream", prec, op, fin.data_ptr(), fout.data_ptr(), None, None, C.byref(g), 0.3, 1.2,
+                 stream())
+        sync()
+        ref = R.op_ns_collide(oprec, rop, dims, (True, True, True), f, tau=0.3, rho0=1.2)
+        assert rel_err(soa_to_aos(host(fout), 19), ref) < 10 * tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+def test_ns_moments(prec, oprec, tol):
+    dims = (11, 6, 5)
+    n = 330
+    say(f"ns_moments prec={prec}")
+    f = _rand_f(n, 4)
+    f[:19] = 0.0  # zero density -> u = 0 branch
+    tdt = _tdt(prec)
+    fin = dev(aos_to_soa(f, 19).ravel(), tdt)
+    outs = [torch().empty(n, dtype=tdt, device="cuda") for _ in range(4)]
+    g = abi.grid(*dims)
+    abi.call("fmx_ns_moments", prec, fin.data_ptr(), *(o.data_ptr() for o in outs), C.byref(g), stream())
+    sync()
+    ref = R.op_ns_moments(oprec, dims, f)
+    for o, r in zip(outs, ref):
+        assert rel_err(host(o), r) < tol
+
+
+def _lists_neq(mask, dims):
+    nx, ny, nz = dims
+    b = np.nonzero(mask)[0].astype(np.int64)
+    x = b % nx
+    step = np.where(x == 0, 1, -1)
+    return b, b + step
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+@pytest.mark.parametrize("plane", [0, -1])
+def test_ns_velocity_neq(prec, oprec, tol, plane):
+    dims = (9, 6, 5)
+    n = 270
+    say(f"ns_bc_neq prec={prec} plane={plane}")
+    f = _rand_f(n, 5)
+    mask = x_plane(*dims, 0 if plane == 0 else dims[0] - 1)
+    bnd, nei = _lists_neq(mask, dims)
+    tdt = _tdt(prec)
+    fd = dev(aos_to_soa(f, 19).ravel(), tdt)
+    bd, nd = dev(bnd, torch().int64), dev(nei, torch().int64)
+    abi.call("fmx_ns_bc_velocity_neq", prec, fd.data_ptr(), bd.data_ptr(), nd.data_ptr(), len(bnd), 0.05, 0.01, -0.02, n,
+             stream())
+    sync()
+    ref = R.op_ns_bc(oprec, R.NS_NEQ, dims, f, mask, u=(0.05, 0.01, -0.02))
+    assert rel_err(soa_to_aos(host(fd), 19), ref) < tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+def test_ns_velocity_eq_pressure_bounceback(prec, oprec, tol):
+    dims = (9, 6, 5)
+    n = 270
+    say(f"ns_bc eq/pressure/bounceback prec={prec}")
+    f = _rand_f(n, 6)
+    tdt = _tdt(prec)
+    inlet, outlet = x_plane(*dims, 0), x_plane(*dims, 8)
+    cells_in = dev(np.nonzero(inlet)[0].astype(np.int64), torch().int64)
+    # Eq velocity
+    fd = dev(aos_to_soa(f, 19).ravel(), tdt)
+    abi.call("fmx_ns_bc_velocity_eq", prec, fd.data_ptr(), cells_in.data_ptr(), cells_in.numel(), 1.1, 0.04, 0.0, 0.01,
+             n, stream())
+    sync()
+    assert rel_err(soa_to_aos(host(fd), 19), R.op_ns_bc(oprec, R.NS_EQ, dims, f, inlet, u=(0.04, 0, 0.01), a=1.1)) < tol
+    # pressure
+    oc = np.nonzero(outlet)[0].astype(np.int64)
+    fd = dev(aos_to_soa(f, 19).ravel(), tdt)
+    c1, c2 = dev(oc, torch().int64), dev(oc - 1, torch().int64)
+    abi.call("fmx_ns_bc_pressure", prec, fd.data_ptr(), c1.data_ptr(), c2.data_ptr(), len(oc), 1.02, n, stream())
+    sync()
+    assert rel_err(soa_to_aos(host(fd), 19), R.op_ns_bc(oprec, R.NS_PRESSURE, dims, f, outlet, a=1.02)) < tol
+    # bounce-back swap on a random mask
+    rng = np.random.default_rng(7)
+    m = (rng.random(n) < 0.4).astype(np.uint8)
+    fd = dev(aos_to_soa(f, 19).ravel(), tdt)
+    cm = dev(np.nonzero(m)[0].astype(np.int64), torch().int64)
+    abi.call("fmx_ns_bc_bounceback", prec, fd.data_ptr(), cm.data_ptr(), cm.numel(), n, stream())
+    sync()
+    assert rel_err(soa_to_aos(host(fd), 19), R.op_ns_bc(oprec, R.NS_BOUNCEBACK, dims, f, m)) < tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+@pytest.mark.parametrize("periodic", [(False, True, True), (True, True, False)])
+def test_ad_collide_stream(prec, oprec, tol, periodic):
+    dims = (12, 7, 6)
+    n = 504
+    say(f"ad_collide prec={prec} periodic={periodic}")
+    rng = np.random.default_rng(8)
+    phi = rng.uniform(0, 1, n)
+    phi[:5] = 1e-12  # clamp branch
+    g = soa_to_aos(np.outer(W7, phi) * (1 + 0.05 * rng.standard_normal((7, n))), 7)
+    tau = rng.uniform(0.52, 1.2, n)
+    u = [rng.uniform(-0.05, 0.05, n) for _ in range(3)]
+    S = rng.uniform(-0.01, 0.01, n)
+    tdt = _tdt(prec)
+    gd = dev(aos_to_soa(g, 7).ravel(), tdt)
+    go = torch().empty_like(gd)
+    ph = torch().empty(n, dtype=tdt, device="cuda")
+    td, ud, sd = dev(tau, tdt), [dev(x, tdt) for x in u], dev(S, tdt)
+    gr = abi.grid(*dims, periodic)
+    abi.call("fmx_ad_collide_stream", prec, gd.data_ptr(), go.data_ptr(), ph.data_ptr(), td.data_ptr(),
+             *(x.data_ptr() for x in ud), sd.data_ptr(), C.byref(gr), 1e-8, stream())
+    sync()
+    rg, rphi = R.op_ad_collide(oprec, dims, periodic, g, tau, *u, S, minC=1e-8)
+    assert rel_err(soa_to_aos(host(go), 7), rg) < tol
+    assert rel_err(host(ph), rphi) < tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+def test_ad_dirichlet_neumann(prec, oprec, tol):
+    dims = (8, 5, 4)
+    n = 160
+    say(f"ad_bc prec={prec}")
+    rng = np.random.default_rng(9)
+    g = rng.random(n * 7)
+    tdt = _tdt(prec)
+    inlet, outlet = x_plane(*dims, 0), x_plane(*dims, 7)
+    gd = dev(aos_to_soa(g, 7).ravel(), tdt)
+    ci = dev(np.nonzero(inlet)[0].astype(np.int64), torch().int64)
+    abi.call("fmx_ad_bc_dirichlet", prec, gd.data_ptr(), ci.data_ptr(), ci.numel(), 0.8, n, stream())
+    sync()
+    assert rel_err(soa_to_aos(host(gd), 7), R.op_ad_bc(oprec, R.AD_DIRICHLET, dims, g, inlet, a=0.8)) < tol
+    oc = np.nonzero(outlet)[0].astype(np.int64)
+    gd = dev(aos_to_soa(g, 7).ravel(), tdt)
+    c1, c2 = dev(oc, torch().int64), dev(oc - 1, torch().int64)
+    abi.call("fmx_ad_bc_neumann", prec, gd.data_ptr(), c1.data_ptr(), c2.data_ptr(), len(oc), n, stream())
+    sync()
+    assert rel_err(soa_to_aos(host(gd), 7), R.op_ad_bc(oprec, R.AD_NEUMANN, dims, g, outlet)) < tol
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+@pytest.mark.parametrize("iso", [abi.ISO_LINEAR, abi.ISO_LANGMUIR])
+def test_ldf(prec, oprec, tol, iso):
+    n = 3000
+    say(f"ldf prec={prec} iso={iso}")
+    rng = np.random.default_rng(10 + iso)
+    region = (rng.random(n) < 0.6).astype(np.uint8)
+    phi = rng.uniform(-0.1, 1.0, n)
+    q = rng.uniform(0, 1, n)
+    S = rng.uniform(-1, 1, n)
+    tdt = _tdt(prec)
+    rd, pd, sd, qd = dev(region, torch().uint8), dev(phi, tdt), dev(S, tdt), dev(q, tdt)
+    if iso == abi.ISO_LINEAR:
+        abi.call("fmx_ldf_linear", prec, n, rd.data_ptr(), pd.data_ptr(), sd.data_ptr(), qd.data_ptr(), 0.5, 0.01, 1e-8,
+                 stream())
+        rs, rq = R.op_ldf(oprec, R.LINEAR, region, phi, S, q, 0.5, 0.0, 0.01)
+    else:
+        abi.call("fmx_ldf_langmuir", prec, n, rd.data_ptr(), pd.data_ptr(), sd.data_ptr(), qd.data_ptr(), 2.0, 5.0,
+                 0.02, 1e-8, stream())
+        rs, rq = R.op_ldf(oprec, R.LANGMUIR, region, phi, S, q, 2.0, 5.0, 0.02)
+    sync()
+    assert rel_err(host(sd), rs) < tol and rel_err(host(qd), rq) < tol
+
+
+@pytest.mark.parametrize("prec", [abi.F32, abi.F64])
+def test_init_pack_unpack(prec):
+    dims = (7, 5, 3)
+    n = 105
+    say(f"init/pack prec={prec}")
+    tdt = _tdt(prec)
+    f = torch().empty(19 * n, dtype=tdt, device="cuda")
+    g = abi.grid(*dims)
+    abi.call("fmx_ns_init_eq", prec, f.data_ptr(), C.byref(g), 1.3, stream())
+    gg = torch().empty(7 * n, dtype=tdt, device="cuda")
+    abi.call("fmx_ad_init_eq", prec, gg.data_ptr(), C.byref(g), 0.25, stream())
+    aos = torch().empty_like(f)
+    abi.call("fmx_pack_aos", prec, f.data_ptr(), aos.data_ptr(), 19, n, stream())
+    back = torch().empty_like(f)
+    abi.call("fmx_unpack_aos", prec, aos.data_ptr(), back.data_ptr(), 19, n, stream())
+    sync()
+    w = W19.astype(np.float32) if prec == abi.F32 else W19
+    assert np.allclose(host(aos), np.tile(w * (np.float32(1.3) if prec == abi.F32 else 1.3), n), rtol=1e-7)
+    assert np.array_equal(host(back), host(f))
+    assert np.allclose(aos_to_soa(soa_to_aos(host(gg), 7), 7).sum(0), 0.25, rtol=1e-7)
