@@ -4,8 +4,8 @@
     python -m fluidmaxxer_amd.build --jobs 8
 
 Objects go to fluidmaxxer_amd/_build/, the library to fluidmaxxer_amd/lib/.
-Device code is compiled without packed-FP32 instructions and with IEEE division
-(no fast-math).
+Device code is compiled without packed-FP32 instructions, without FMA
+contraction and with IEEE division (no fast-math).
 """
 from __future__ import annotations
 
@@ -32,6 +32,9 @@ CFLAGS = [
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # no FMA contraction: every kernel performs the reference's IEEE operations in the
+    # reference's order, so FP32/FP64 results are reproducible against the CPU restatement
+    "-ffp-contract=off",
     "-Wall",
     "-Wno-unused-variable",
     "-Wno-unused-command-line-argument",
@@ -72,8 +75,27 @@ def build(jobs: int = 0) -> str:
     return LIB
 
 
+def build_cpp_driver() -> list:
+    """The C++ API driver of the test suite (tests/cpp/coupled_driver.cpp), float and double."""
+    src = os.path.join(ROOT, "tests", "cpp", "coupled_driver.cpp")
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for dt in ("float", "double"):
+        exe = os.path.join(out_dir, f"coupled_driver_{dt}")
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-DFMX_DATA_TYPE={dt}", f"-I{os.path.join(ROOT, 'include')}",
+               f"-I{os.path.join(PKG, 'host')}", src, "-o", exe, f"-L{LIBDIR}", "-lfmx_hip", f"-Wl,-rpath,{LIBDIR}",
+               "-Wl,-rpath,$ORIGIN/../../../fluidmaxxer_amd/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed:\n{r.stderr}")
+        outs.append(exe)
+    return outs
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=0)
     a = ap.parse_args()
     print(build(a.jobs))
+    print(*build_cpp_driver(), sep="\n")

@@ -36,7 +36,7 @@ class CoupledConfig:
     rho0: float = 1.0
     tau: float = 0.8
     u_in: tuple = (0.05, 0.0, 0.0)
-    ns_inlet: bool = True                  # VelocityDirichletNEQ on x = 0
+    ns_inlet: Optional[str] = "eq"         # "eq": VelocityDirichletEq, "neq": VelocityDirichletNEQ, on x = 0
     ns_outlet: bool = True                 # PressureDirichlet on x = nx-1
     rho_out: float = 1.0
     field: Optional[np.ndarray] = None     # porosity (porous) / solid fraction (pbb), float[N]
@@ -71,7 +71,8 @@ class CoupledConfig:
         d.periodic_y, d.periodic_z = int(self.periodic_y), int(self.periodic_z)
         d.prec = prec
         d.ns_op = _NS_OPS[self.ns_op]
-        d.ns_inlet, d.ns_outlet = int(self.ns_inlet), int(self.ns_outlet)
+        d.ns_inlet = {None: 0, "none": 0, "neq": 1, "eq": 2}[self.ns_inlet]
+        d.ns_outlet = int(self.ns_outlet)
         d.ad_inlet = {"none": 0, None: 0, "dirichlet": 1, "series": 2}[self.ad_inlet]
         d.ad_outlet = int(self.ad_outlet)
         d.isotherm = _ISO[self.isotherm]
@@ -117,7 +118,8 @@ class CoupledSim:
         self.flags = torch.from_numpy(cfg.flags()).to(dev)
         self.field = None if cfg.field is None else torch.as_tensor(cfg.field, dtype=self.cdt).to(dev)
         self.tau_field = None if cfg.tau_field is None else torch.as_tensor(cfg.tau_field, dtype=self.cdt).to(dev)
-        self.inlet_raw = torch.zeros(19 * cfg.ny * cfg.nz, dtype=self.cdt, device=dev) if cfg.ns_inlet else None
+        has_inlet = cfg.ns_inlet not in (None, "none")
+        self.inlet_raw = torch.zeros(19 * cfg.ny * cfg.nz, dtype=self.cdt, device=dev) if has_inlet else None
         self.inlet_series = None
         if cfg.ad_inlet == "series":
             if cfg.inlet_series is None:

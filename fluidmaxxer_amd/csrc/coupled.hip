@@ -26,6 +26,9 @@
 //     is what the reference holds in f(0,y,z) between steps (used for the AD
 //     velocity here and for state read-back).  fmx_coupled_prologue applies the
 //     first step's update.
+//   * Equilibrium velocity inlet (VelocityDirichletEq, VelocityDirichlet.cu:54-84):
+//     the x = 0 thread stores feq(rho0, u_BC) as the next step's f(0,y,z) and its
+//     own post-collision populations into `inlet_raw`.
 //   * ADDirichlet inlet (post-stream): the x = 0 thread stores phi_in * w.
 //   * Pre-collision inlet series (ADDirichletTanhPulse pattern): every pull whose
 //     source cell (or keep-local cell) lies on x = 0 uses phi_in(step) * w_q.
@@ -33,6 +36,7 @@
 //     in place as state (read, then overwritten with this step's value).
 //   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
 //     into FMX_OUTLET_SLOTS FP64 accumulators per step.
+#include "fmx_exact.hpp"
 #include "fmx_internal.hpp"
 
 namespace fmx {
@@ -78,10 +82,49 @@ __device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, 
     bz = int(r / unsigned(gy));
 }
 
+// Population plane q of a structure-of-arrays buffer (wave-uniform pointer) and a load /
+// store at a 32-bit byte offset inside it, so the address is a scalar base plus a vector
+// offset (the host guarantees every plane is below 4 GiB).
+template <class T> __device__ __forceinline__ T *plane(T *base, int q, i64 n) { return base + i64(q) * n; }
+template <class T> __device__ __forceinline__ T ld_b(const T *base, unsigned boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T> __device__ __forceinline__ void st_b(T *base, unsigned boff, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+template <class T> __device__ __forceinline__ void st_b_nt(T *base, unsigned boff, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
+}
+
+template <int P>
+__device__ __forceinline__ typename Storage<P>::C decode(typename Storage<P>::S v, typename Storage<P>::C w,
+                                                          typename Storage<P>::C ref) {
+    using C = typename Storage<P>::C;
+    if constexpr (IsHalf<P>::value) return w * (ref + C(v));
+    else return v;
+}
+template <int P>
+__device__ __forceinline__ typename Storage<P>::S encode(typename Storage<P>::C v, typename Storage<P>::C inv_w,
+                                                          typename Storage<P>::C ref) {
+    using S = typename Storage<P>::S;
+    if constexpr (IsHalf<P>::value) return S(v * inv_w - ref);
+    else return v;
+}
+
+// feq in the BGK form w rho (1 + cu + 0.5 cu^2 - 1.5 u^2), cu = 3 e.u (BGKCollision.cu:166-173)
+template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy, C uz, C u2x15) {
+    const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
+    const C cu = C(3) * eu;
+    return w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - u2x15);
+}
+
 template <int P, int OP>
 __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 acc_row) {
     using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    constexpr unsigned SB = sizeof(S);
+    constexpr unsigned CB = sizeof(C);
     int bx, by, bz;
     block_coords(prm.xcd_remap, prm.gx, prm.gy, prm.gz, bx, by, bz);
     const int x = bx * prm.block_x + int(threadIdx.x);
@@ -90,11 +133,12 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     if (x >= prm.nx || y >= prm.ny) return;
     const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
     const i64 n = prm.n;
-    const i64 cell = cell_index(x, y, z, nx, ny);
+    const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
+    const unsigned cell_b = cell * SB;
     const std::uint8_t fl = p.flags[cell];
     const bool solid = (OP == FMX_NS_BGK) && (fl & FMX_FLAG_SOLID);
 
-    // neighbour coordinates: x keeps local out of domain, y/z wrap when periodic
+    // neighbour rows: x keeps local out of domain, y/z wrap when periodic
     const bool xm_ok = x > 0, xp_ok = x < nx - 1;
     int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
     bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
@@ -108,36 +152,42 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         if (!zp_ok) zp -= nz;
         zm_ok = zp_ok = true;
     }
-    const int ys[3] = {ym, y, yp};
-    const int zs[3] = {zm, z, zp};
+    const unsigned ys[3] = {unsigned(ym), unsigned(y), unsigned(yp)};
+    const unsigned zs[3] = {unsigned(zm), unsigned(z), unsigned(zp)};
     const bool yok[3] = {ym_ok, true, yp_ok};
     const bool zok[3] = {zm_ok, true, zp_ok};
     const bool xok[3] = {xm_ok, true, xp_ok};
+    // source cell of the pull along (cx, cy, cz) = cell - c, or the cell itself when out of domain
+    auto src_of = [&](int cx, int cy, int cz) -> unsigned {
+        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
+        return ok ? unsigned(x - cx) + unsigned(nx) * (ys[1 - cy] + unsigned(ny) * zs[1 - cz]) : cell;
+    };
 
     // ---------------------------------------------------------------- NS pull
     C fo[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-        const int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
-        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
-        const i64 src = ok ? cell_index(x - cx, ys[1 - cy], zs[1 - cz], nx, ny) : cell;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
+        const unsigned src = src_of(cx, cy, cz);
         bool bb = solid;
-        if (OP == FMX_NS_BGK && q != 0 && !solid && ok) bb = (p.flags[src] & FMX_FLAG_SOLID) != 0;
-        const i64 addr = bb ? i64(Q19::opp[q]) * n + cell : i64(q) * n + src;
-        fo[q] = ld_pop<P>(p.f, addr, w19<C>(q), prm.f_ref);
-    }
+        if constexpr (OP == FMX_NS_BGK && q != 0)
+            if (!solid && src != cell) bb = (p.flags[src] & FMX_FLAG_SOLID) != 0;
+        S raw;
+        if (bb) raw = ld_b(plane(p.f, Q19::opp[q], n), cell_b);
+        else raw = ld_b(plane(p.f, q, n), src * SB);
+        fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
+    });
 
     // ----------------------------------------------------- NS collide (in place)
     {
         C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-            const C v = fo[q];
-            rho += v;
-            jx += v * C(Q19::cx[q]);
-            jy += v * C(Q19::cy[q]);
-            jz += v * C(Q19::cz[q]);
-        }
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            rho += fo[q];
+            jx = axpy1<Q19::cx[q]>(jx, fo[q]);
+            jy = axpy1<Q19::cy[q]>(jy, fo[q]);
+            jz = axpy1<Q19::cz[q]>(jz, fo[q]);
+        });
         C ux, uy, uz;
         if (OP == FMX_NS_POROUS) {  // PorousBGKCollision.cu:118-136
             if (!(rho > C(0))) {
@@ -164,23 +214,23 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
                 uz = jz * invR;
             }
         }
-        const C u2 = ux * ux + uy * uy + uz * uz;
+        const C u2x15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
         C ns = C(0);
         if (OP == FMX_NS_PBB) {
             ns = p.field ? p.field[cell] : C(0);
             ns = ns < C(0) ? C(0) : (ns > C(1) ? C(1) : ns);
         }
         C fc[19];
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-            const C eu = C(Q19::cx[q]) * ux + C(Q19::cy[q]) * uy + C(Q19::cz[q]) * uz;
-            const C cu = C(3) * eu;
-            const C feq = w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - C(1.5) * u2);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const C feq = feq_bgk<q>(rho, ux, uy, uz, u2x15);
             fc[q] = fo[q] + prm.omega * (feq - fo[q]);
-        }
+        });
         if (OP == FMX_NS_PBB) {
-#pragma unroll
-            for (int q = 0; q < 19; ++q) fc[q] = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                fc[q] = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
+            });
         }
         if (!solid) {
 #pragma unroll
@@ -190,13 +240,13 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
 
     // ------------------------------------- post-collision moments (Macros.cu:44-60)
     C r2 = C(0), vx = C(0), vy = C(0), vz = C(0);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         r2 += fo[q];
-        vx += fo[q] * C(Q19::cx[q]);
-        vy += fo[q] * C(Q19::cy[q]);
-        vz += fo[q] * C(Q19::cz[q]);
-    }
+        vx = axpy1<Q19::cx[q]>(vx, fo[q]);
+        vy = axpy1<Q19::cy[q]>(vy, fo[q]);
+        vz = axpy1<Q19::cz[q]>(vz, fo[q]);
+    });
     const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
     const C ux2 = vx * inv2, uy2 = vy * inv2, uz2 = vz * inv2;
 
@@ -204,41 +254,47 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     const bool inlet_cell = prm.ns_inlet && x == 0;
     const bool outlet_cell = prm.ns_outlet && x == nx - 1;
     if (inlet_cell) {
-        const i64 np = i64(ny) * nz, pc = i64(y) + i64(ny) * z;
+        const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
 #pragma unroll
-        for (int q = 0; q < 19; ++q) p.inlet_raw[q * np + pc] = fo[q];
+        for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+        if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
+            const C b15 = C(1.5) * prm.uin2;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C v = feq_bgk<q>(prm.rho0, prm.uin_x, prm.uin_y, prm.uin_z, b15);
+                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+            });
+        }
     } else if (!outlet_cell) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) st_pop_nt<P>(p.fn, i64(q) * n + cell, fo[q], C(Q19::inv_w[q]), prm.f_ref);
+        for (int q = 0; q < 19; ++q)
+            st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
     }
-    if (prm.ns_inlet && x == 1) {  // next step's NEQ update of (0, y, z)
-        const C u2N = ux2 * ux2 + uy2 * uy2 + uz2 * uz2;
-        const i64 dst = cell - 1;
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-            const C euN = C(Q19::cx[q]) * ux2 + C(Q19::cy[q]) * uy2 + C(Q19::cz[q]) * uz2;
-            const C cuN = C(3) * euN;
-            const C feqN = w19<C>(q) * r2 * (C(1) + cuN + C(0.5) * cuN * cuN - C(1.5) * u2N);
-            const C euB = C(Q19::cx[q]) * prm.uin_x + C(Q19::cy[q]) * prm.uin_y + C(Q19::cz[q]) * prm.uin_z;
-            const C cuB = C(3) * euB;
-            const C feqB = w19<C>(q) * r2 * (C(1) + cuB + C(0.5) * cuB * cuB - C(1.5) * prm.uin2);
-            st_pop<P>(p.fn, i64(q) * n + dst, feqB + (fo[q] - feqN), C(Q19::inv_w[q]), prm.f_ref);
-        }
+    if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z) (VelocityDirichletNEQ.cu:81-95)
+        const C u2N = C(1.5) * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2);
+        const C u2B = C(1.5) * prm.uin2;
+        const unsigned dst_b = cell_b - SB;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const C feqN = feq_bgk<q>(r2, ux2, uy2, uz2, u2N);
+            const C feqB = feq_bgk<q>(r2, prm.uin_x, prm.uin_y, prm.uin_z, u2B);
+            st_b(plane(p.fn, q, n), dst_b, encode<P>(feqB + (fo[q] - feqN), C(Q19::inv_w[q]), prm.f_ref));
+        });
     }
-    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
-        const C u2o = ux2 * ux2 + uy2 * uy2 + uz2 * uz2;
-        const i64 dst = cell + 1;
-#pragma unroll
-        for (int q = 0; q < 19; ++q) {
-            const C eu = C(Q19::cx[q]) * ux2 + C(Q19::cy[q]) * uy2 + C(Q19::cz[q]) * uz2;
-            const C v = w19<C>(q) * prm.rho_out * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * u2o);
-            st_pop<P>(p.fn, i64(q) * n + dst, v, C(Q19::inv_w[q]), prm.f_ref);
-        }
+    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z) (PressureDirichlet.cu:81-95)
+        const C u2o = C(1.5) * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2);
+        const unsigned dst_b = cell_b + SB;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux2, uy2, uz2);
+            const C v = w19<C>(q) * prm.rho_out * (C(1) + C(3) * eu + C(4.5) * eu * eu - u2o);
+            st_b(plane(p.fn, q, n), dst_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+        });
     }
 
     // ------------------------------------------------------ sorption source
-    C src = C(0);
-    if (prm.isotherm && (fl & FMX_FLAG_REGION)) {
+    C src_term = C(0);
+    if (prm.isotherm && (fl & FMX_FLAG_REGION)) {  // LinearLDF.cu:27-47, LangmuirLDF.cu:23-44
         C c = p.phi[cell];
         if (c < prm.min_conc) c = prm.min_conc;
         const C ce = c - prm.min_conc;
@@ -252,21 +308,20 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         const C qi = p.q[cell];
         const C dq = prm.k_ldf * (qeq - qi);
         p.q[cell] = qi + dq;
-        src = -dq;
+        src_term = -dq;
     }
 
     // --------------------------------------------------------------- AD pull
     const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
     C gi[7];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        const int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
-        const int sx = ok ? x - cx : x;
-        const i64 s = ok ? cell_index(sx, ys[1 - cy], zs[1 - cz], nx, ny) : cell;
-        if (prm.ad_inlet == 2 && sx == 0) gi[q] = phi_pre * w7<C>(q);
-        else gi[q] = ld_pop<P>(p.g, i64(q) * n + s, w7<C>(q), prm.g_ref);
-    }
+    static_for<0, 7>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+        const unsigned s = src_of(cx, cy, cz);
+        const bool on_inlet = (cx == 1) ? (x <= 1) : (x == 0);  // source (or keep-local) cell on x = 0
+        if (prm.ad_inlet == 2 && on_inlet) gi[q] = phi_pre * w7<C>(q);
+        else gi[q] = decode<P>(ld_b(plane(p.g, q, n), s * SB), w7<C>(q), prm.g_ref);
+    });
     C phi = C(0);
 #pragma unroll
     for (int q = 0; q < 7; ++q) phi += gi[q];
@@ -274,34 +329,36 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
     const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
     C go[7];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        const C eu = C(Q7::cx[q]) * ax + C(Q7::cy[q]) * ay + C(Q7::cz[q]) * az;
+    static_for<0, 7>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const C eu = dot3<Q7::cx[q], Q7::cy[q], Q7::cz[q]>(ax, ay, az);
         const C geq = w7<C>(q) * phi * (C(1) + C(4) * eu);
-        go[q] = gi[q] + om * (geq - gi[q]) + w7<C>(q) * src;
-    }
+        go[q] = gi[q] + om * (geq - gi[q]) + w7<C>(q) * src_term;
+    });
     p.phi[cell] = phi;
 
     // ------------------------------------------------------------- AD stores
     const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
     if (prm.ad_inlet == 1 && x == 0) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_pop_nt<P>(p.gn, i64(q) * n + cell, prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), prm.g_ref);
+        for (int q = 0; q < 7; ++q)
+            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), prm.g_ref));
     } else if (!ad_out_cell) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_pop_nt<P>(p.gn, i64(q) * n + cell, go[q], C(Q7::inv_w[q]), prm.g_ref);
+        for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), prm.g_ref));
     }
     if (prm.ad_outlet && x == nx - 2) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_pop<P>(p.gn, i64(q) * n + cell + 1, go[q], C(Q7::inv_w[q]), prm.g_ref);
+        for (int q = 0; q < 7; ++q) st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), prm.g_ref));
     }
 
     // ------------------------------------------------- outlet breakthrough sum
     if (p.outlet_acc && x == nx - 1) {
-        const int slot = int((i64(y) + i64(ny) * z) % FMX_OUTLET_SLOTS);
+        const int slot = int((unsigned(y) + unsigned(ny) * unsigned(z)) % FMX_OUTLET_SLOTS);
         unsafeAtomicAdd(p.outlet_acc + acc_row * (FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE) + slot * FMX_OUTLET_STRIDE,
                         double(phi));
     }
+    (void)CB;
 }
 
 // First step's inlet update (NEQ on f(0,y,z) from f(1,y,z)) and the raw x = 0 plane.
@@ -327,7 +384,17 @@ __global__ void __launch_bounds__(256) k_coupled_prologue(typename Storage<P>::S
         jy += f1[q] * C(Q19::cy[q]);
         jz += f1[q] * C(Q19::cz[q]);
     }
-    if (!prm.ns_inlet) return;
+    if (prm.ns_inlet == 2) {  // VelocityDirichletEq (VelocityDirichlet.cu:54-84)
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const C euB = C(Q19::cx[q]) * prm.uin_x + C(Q19::cy[q]) * prm.uin_y + C(Q19::cz[q]) * prm.uin_z;
+            const C cuB = C(3) * euB;
+            const C v = w19<C>(q) * prm.rho0 * (C(1) + cuB + C(0.5) * cuB * cuB - C(1.5) * prm.uin2);
+            st_pop<P>(f, i64(q) * n + c0, v, C(Q19::inv_w[q]), prm.f_ref);
+        }
+        return;
+    }
+    if (prm.ns_inlet != 1) return;
     const C inv = (r > C(0)) ? (C(1) / r) : C(0);
     const C ux = jx * inv, uy = jy * inv, uz = jz * inv;
     const C u2N = ux * ux + uy * uy + uz * uz;
@@ -563,10 +630,12 @@ int fmx_coupled_validate(const fmx_coupled_desc *d, const fmx_coupled_bufs *b) {
         return set_error(FMX_ERR_ARG, "fmx_coupled: need nx >= 3, ny >= 1, 1 <= nz < 65536 (got %d x %d x %d)", d->nx,
                          d->ny, d->nz);
     const i64 n = i64(d->nx) * d->ny * d->nz;
-    if (n > (i64(1) << 40)) return set_error(FMX_ERR_ARG, "fmx_coupled: domain too large");
+    const i64 sb = d->prec == FMX_F64 ? 8 : 4;  // widest per-cell element the kernel addresses
+    if (n * sb >= (i64(1) << 32))
+        return set_error(FMX_ERR_ARG, "fmx_coupled: %lld cells exceed the 4 GiB population-plane limit", (long long)n);
     if (d->prec < FMX_F32 || d->prec > FMX_BF16) return set_error(FMX_ERR_ARG, "fmx_coupled: bad precision %d", d->prec);
     if (d->ns_op < FMX_NS_BGK || d->ns_op > FMX_NS_PBB) return set_error(FMX_ERR_ARG, "fmx_coupled: bad ns_op");
-    if (d->ns_inlet < 0 || d->ns_inlet > 1 || d->ns_outlet < 0 || d->ns_outlet > 1 || d->ad_inlet < 0 ||
+    if (d->ns_inlet < 0 || d->ns_inlet > 2 || d->ns_outlet < 0 || d->ns_outlet > 1 || d->ad_inlet < 0 ||
         d->ad_inlet > 2 || d->ad_outlet < 0 || d->ad_outlet > 1 || d->isotherm < 0 || d->isotherm > 2)
         return set_error(FMX_ERR_ARG, "fmx_coupled: bad boundary/isotherm selector");
     if (d->ns_outlet && !d->ad_outlet)

@@ -10,6 +10,13 @@ C4  128x64x64, porosity 0.5, C2 physics with partial bounce-back solid fraction,
 C5  256x128x128, porosity 0.4, inlet concentration series of 200 knots, partial
     bounce-back, 20000 steps (differentiation + precision)
 
+Inlet: VelocityDirichletEq (VelocityDirichlet.cu:54-84), which holds the inlet
+density at rho0.  main.cpp's VelocityDirichletNEQ takes the density from the
+neighbour cell (VelocityDirichletNEQ.cu:86-94); against these sphere packs that
+feeds back -- the inflow rho_N u_in grows with the pressure it builds -- and the
+inlet density diverges geometrically (C1: x10 in 1000 steps), so it is available
+but not used by the workloads.
+
 Walls: solid layers y in {0, ny-1} and z in {0, nz-1}; periodic y/z wrap as in
 main.cpp:143-145 (the walls make it immaterial for the NS lattice).  Sorption
 region: sphere voxels.  Porosity = fluid cells / all cells, walls counted as
@@ -73,7 +80,7 @@ def build(name: str, dims=None, steps=None) -> CoupledConfig:
     ns_op = spec.get("ns_op", "bgk")
     cfg = CoupledConfig(
         nx=nx, ny=ny, nz=nz, solid=pack.solid, region=pack.sphere, ns_op=ns_op,
-        tau=spec["tau"], u_in=(spec["u_in"], 0.0, 0.0), tau_ad=spec["tau_ad"],
+        tau=spec["tau"], u_in=(spec["u_in"], 0.0, 0.0), tau_ad=spec["tau_ad"], ns_inlet="eq",
         isotherm=spec["isotherm"], iso_a=spec["iso_a"], iso_b=spec["iso_b"], k_ldf=spec["k_ldf"],
     )
     nsteps = steps if steps is not None else spec["steps"]

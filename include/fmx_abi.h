@@ -141,7 +141,7 @@ typedef struct fmx_coupled_desc {
     int32_t periodic_y, periodic_z; /* x is never periodic */
     int32_t prec;                   /* population storage, FMX_F32/F64/F16/BF16 */
     int32_t ns_op;                  /* FMX_NS_* */
-    int32_t ns_inlet;               /* 0 none, 1 VelocityDirichletNEQ at x = 0 */
+    int32_t ns_inlet;               /* 0 none, 1 VelocityDirichletNEQ, 2 VelocityDirichletEq, at x = 0 */
     int32_t ns_outlet;              /* 0 none, 1 PressureDirichlet at x = nx-1 (requires ad_outlet = 1) */
     int32_t ad_inlet;               /* 0 none, 1 ADDirichlet (post-stream) at x = 0,
                                        2 pre-collision Dirichlet at x = 0 with a per-step value */
@@ -169,7 +169,7 @@ typedef struct fmx_coupled_bufs {
     const uint8_t *flags;     /* N: FMX_FLAG_SOLID | FMX_FLAG_REGION */
     const void *field;        /* N porosity (POROUS) or solid fraction (PBB); NULL = 1 / 0 */
     const void *tau_field;    /* N AD relaxation times; NULL = tau_ad */
-    void *inlet_raw;          /* 19*ny*nz, required with ns_inlet = 1: the x = 0 plane's populations as the
+    void *inlet_raw;          /* 19*ny*nz, required with ns_inlet != 0: the x = 0 plane's populations as the
                                  reference holds them between steps (before the next inlet update) */
     const double *inlet_series; /* ad_inlet = 2: inlet value of step s at inlet_series[s] */
     double *outlet_acc;       /* nullable: FMX_OUTLET_SLOTS*FMX_OUTLET_STRIDE doubles per step, accumulating

@@ -47,8 +47,10 @@ def oracle_setup(cfg, R):
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     inlet, outlet = x_plane(nx, ny, nz, 0), x_plane(nx, ny, nz, nx - 1)
     ns_bc = []
-    if cfg.ns_inlet:
+    if cfg.ns_inlet == "neq":
         ns_bc.append(R.BC(R.NS_NEQ, inlet, tuple(cfg.u_in)))
+    elif cfg.ns_inlet == "eq":
+        ns_bc.append(R.BC(R.NS_EQ, inlet, tuple(cfg.u_in)))
     if cfg.ns_outlet:
         ns_bc.append(R.BC(R.NS_PRESSURE, outlet, (cfg.rho_out,)))
     ad_bc = []

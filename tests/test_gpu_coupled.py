@@ -26,7 +26,7 @@ def small_config(variant, dims=(24, 12, 10), seed=11):
     nx, ny, nz = dims
     pack = sphere_pack(nx, ny, nz, seed, 2.0, 3.0, 0.6, overlap=False, clearance=4)
     cfg = CoupledConfig(nx, ny, nz, solid=pack.solid, region=pack.sphere, tau=0.8, u_in=(0.05, 0.0, 0.0),
-                        tau_ad=0.7, isotherm="linear", iso_a=0.5, k_ldf=0.05)
+                        tau_ad=0.7, isotherm="linear", iso_a=0.5, k_ldf=0.05, ns_inlet="neq")
     rng = np.random.default_rng(seed)
     n = cfg.n
     if variant == "porous":
@@ -46,8 +46,11 @@ def small_config(variant, dims=(24, 12, 10), seed=11):
     elif variant == "closed_walls":
         cfg.periodic_y = cfg.periodic_z = False
         cfg.tau_field = np.where(pack.solid == 1, 0.9, 0.6)
+    elif variant == "eq_inlet":
+        cfg.ns_inlet = "eq"
     elif variant == "no_inlet_outlet":
-        cfg.ns_inlet = cfg.ns_outlet = False
+        cfg.ns_inlet = None
+        cfg.ns_outlet = False
         cfg.ad_outlet = False
         cfg.ad_inlet = "none"
         cfg.phi0 = 0.4
@@ -85,12 +88,13 @@ def run_pair(cfg, prec, steps, perturb=True, seed=5):
                q=orc.get(R.Q), g=aos_to_soa(orc.get(R.G), 7).ravel(), c_out=c_ref)
     errs = {k: rel_err(got[k], ref[k]) for k in got}
     say("  errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    say("  max |diff|:", {k: f"{np.max(np.abs(np.asarray(got[k]) - np.asarray(ref[k]))):.1e}" for k in got})
     return errs
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("variant", ["bgk", "porous", "pbb", "langmuir_series", "uncoupled", "closed_walls",
-                                     "no_inlet_outlet"])
+@pytest.mark.parametrize("variant", ["bgk", "eq_inlet", "porous", "pbb", "langmuir_series", "uncoupled",
+                                     "closed_walls", "no_inlet_outlet"])
 def test_fused_step_matches_oracle(prec, variant):
     cfg = small_config(variant)
     errs = run_pair(cfg, prec, 150)
@@ -99,14 +103,15 @@ def test_fused_step_matches_oracle(prec, variant):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("dims", [(3, 5, 4), (4, 3, 3), (5, 1, 7)])
+@pytest.mark.parametrize("dims", [(3, 5, 4), (4, 3, 3), (5, 2, 7)])
 def test_fused_step_small_extents(dims):
     """Smallest supported boxes: the x = 1 and x = nx-2 threads carry both boundary updates."""
     nx, ny, nz = dims
     n = nx * ny * nz
     rng = np.random.default_rng(3)
     cfg = CoupledConfig(nx, ny, nz, solid=(rng.random(n) < 0.2).astype(np.uint8),
-                        region=(rng.random(n) < 0.5).astype(np.uint8), tau=0.9, tau_ad=0.8, k_ldf=0.1)
+                        region=(rng.random(n) < 0.5).astype(np.uint8), tau=0.9, tau_ad=0.8, k_ldf=0.1,
+                        ns_inlet="neq")
     errs = run_pair(cfg, "f64", 40)
     assert max(errs.values()) < 1e-11, errs
 
@@ -118,6 +123,15 @@ def test_c1_full_run(prec):
     errs = run_pair(cfg, prec, 1000, perturb=False)
     tol = TOL[prec][1]
     assert all(v <= tol for v in errs.values()), errs
+
+
+def test_c1_neq_inlet_short_run():
+    """C1 with main.cpp's NEQ velocity inlet: its inlet density diverges on this pack (x10 by step 1000, in
+    the oracle too), so the comparison stops at 400 steps, before the flow turns chaotic."""
+    cfg = workloads.build("C1")
+    cfg.ns_inlet = "neq"
+    errs = run_pair(cfg, "f32", 400, perturb=False)
+    assert all(v <= 1e-5 for v in errs.values()), errs
 
 
 def test_c1_fp32_against_fp64_oracle():
@@ -136,21 +150,30 @@ def test_c1_fp32_against_fp64_oracle():
     assert all(v < 1e-4 for v in errs.values()), errs
 
 
-@pytest.mark.parametrize("prec", ["f16", "bf16"])
-def test_half_storage_against_fp64(prec):
-    """FP16 / BF16 deviation storage with FP32 arithmetic against the FP64 oracle on C1 (errors reported)."""
-    cfg = workloads.build("C1")
-    say(f"C1 {prec} vs f64 oracle")
-    sim = CoupledSim(cfg, prec=prec)
+@pytest.fixture(scope="module")
+def breakthrough_ref():
+    """C1 physics on a 32x24x24 pack for 3000 steps: the front reaches the outlet (C_out ~ 0.41)."""
+    cfg = workloads.build("C1", dims=(32, 24, 24))
     orc = R.System(oracle_setup(cfg, R), 8)
     outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
-    c = host(sim.run(1000))
-    cr, _, _, _ = orc.run(1000, outlet)
+    c, _, _, _ = orc.run(3000, outlet)
+    ref = dict(rho=orc.get(R.RHO), ux=orc.get(R.UX), phi=orc.get(R.PHI), q=orc.get(R.Q), c_out=c)
+    return cfg, ref
+
+
+@pytest.mark.parametrize("prec,bound", [("f32", 1e-5), ("f16", 2e-2), ("bf16", 5e-2)])
+def test_storage_precision_against_fp64(breakthrough_ref, prec, bound):
+    """Population storage precision against the FP64 oracle over a breakthrough (errors reported per field;
+    the breakthrough error is max_t |dC_out(t)| / max_t C_out(t))."""
+    cfg, ref = breakthrough_ref
+    say(f"breakthrough {prec} vs f64 oracle")
+    sim = CoupledSim(cfg, prec=prec)
+    c = host(sim.run(3000))
     rho, ux, _, _ = (host(t) for t in sim.moments())
-    errs = dict(rho=rel_err(rho, orc.get(R.RHO)), ux=rel_err(ux, orc.get(R.UX)), phi=rel_err(host(sim.phi), orc.get(R.PHI)),
-                q=rel_err(host(sim.q), orc.get(R.Q)), c_out=rel_err(c, cr))
+    got = dict(rho=rho, ux=ux, phi=host(sim.phi), q=host(sim.q), c_out=c)
+    errs = {k: rel_err(got[k], ref[k]) for k in ref}
     say(f"  {prec} vs f64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert errs["c_out"] < 5e-2, errs
+    assert errs["c_out"] < bound, errs
 
 
 def test_mass_reduction_matches_host_sums():
