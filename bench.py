@@ -65,6 +65,44 @@ def cpu_baseline(cfg, prec, budget_s):
     }
 
 
+def stream_probe(gib=2, reps=10):
+    """Achievable HBM bandwidth of plain streaming kernels on this GPU (the practical ceiling)."""
+    import torch
+
+    from fluidmaxxer_amd import abi
+    n = gib * (1 << 30) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    nbytes = n * 4
+    out["copy_GBps"] = round(2 * nbytes / timed(lambda: b.copy_(a)) / 1e9, 1)
+    out["runtime_copy_GBps"] = round(2 * nbytes / timed(
+        lambda: abi.call("fmx_memcpy_d2d", b.data_ptr(), a.data_ptr(), nbytes, torch.cuda.current_stream().cuda_stream)) / 1e9, 1)
+    s = torch.cuda.current_stream().cuda_stream
+    out["stream_copy_GBps"] = round(2 * nbytes / timed(
+        lambda: abi.call("fmx_stream_probe", 0, b.data_ptr(), a.data_ptr(), nbytes, s)) / 1e9, 1)
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    out["stream_read_GBps"] = round(nbytes / timed(
+        lambda: abi.call("fmx_stream_probe", 1, sink.data_ptr(), a.data_ptr(), nbytes, s)) / 1e9, 1)
+    out["torch_sum_read_GBps"] = round(nbytes / timed(lambda: a.sum()) / 1e9, 1)
+    out["write_GBps"] = round(nbytes / timed(
+        lambda: abi.call("fmx_fill_zero", b.data_ptr(), nbytes, torch.cuda.current_stream().cuda_stream)) / 1e9, 1)
+    out["bytes"] = nbytes
+    return {"stream_probe": out}
+
+
 def traffic_from_profile(workload, prec):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload.lower()}_{prec}.json")
     if not os.path.exists(path):
@@ -85,6 +123,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-outlet", action="store_true", help="skip the per-step outlet reduction")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device-to-device copy of BYTES (HBM-counter calibration)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="instead of the benchmark, measure plain streaming bandwidth (copy / read / write)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,6 +139,19 @@ def main():
 
     from fluidmaxxer_amd import workloads
     from fluidmaxxer_amd.coupled import CoupledSim
+
+    if args.stream_probe:
+        print(json.dumps(stream_probe()), flush=True)
+        return
+
+    if args.calibrate_copy:
+        from fluidmaxxer_amd import abi
+        src = torch.empty(args.calibrate_copy // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        abi.call("fmx_memcpy_d2d", dst.data_ptr(), src.data_ptr(), src.numel() * 4, None)
+        torch.cuda.synchronize()
+        del src, dst
 
     dims = tuple(int(v) for v in args.dims.split("x")) if args.dims else None
     spec_steps = workloads.SPECS[args.workload]["steps"]

@@ -46,15 +46,15 @@ class CoupledDesc(C.Structure):
         ("u_in", C.c_double * 3),
         ("rho_out", C.c_double), ("phi_in", C.c_double), ("tau_ad", C.c_double), ("min_conc", C.c_double),
         ("iso_a", C.c_double), ("iso_b", C.c_double), ("k_ldf", C.c_double),
-        ("f_ref", C.c_double), ("g_ref", C.c_double),
+        ("f_ref", C.c_double),
     ]
 
 
 class CoupledBufs(C.Structure):
     _fields_ = [
         ("f", C.c_void_p * 2), ("g", C.c_void_p * 2),
-        ("q", C.c_void_p), ("phi", C.c_void_p),
-        ("flags", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
+        ("q", C.c_void_p), ("phi", C.c_void_p * 2),
+        ("links", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
         ("inlet_raw", C.c_void_p), ("inlet_series", C.c_void_p), ("outlet_acc", C.c_void_p),
     ]
 
@@ -100,6 +100,7 @@ _SIGS = {
     "fmx_ldf_linear": ([_i, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _vp], _i),
     "fmx_ldf_langmuir": ([_i, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp], _i),
     "fmx_coupled_validate": ([_DP, _BP], _i),
+    "fmx_coupled_build_links": ([_DP, _vp, _vp, _vp], _i),
     "fmx_coupled_init": ([_DP, _BP, _i, _d, _vp], _i),
     "fmx_coupled_load": ([_DP, _BP, _i, _vp, _vp, _vp, _vp, _vp], _i),
     "fmx_coupled_prologue": ([_DP, _BP, _i, _vp], _i),
@@ -108,6 +109,7 @@ _SIGS = {
     "fmx_coupled_view_g": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_outlet_mean": ([_vp, _i64, _i64, _vp, _vp], _i),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
+    "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,
                               C.c_int64, _vp, _vp, C.POINTER(C.c_double)], _i),
     "fmx_build_x_plane_mask": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp], _i),

@@ -54,7 +54,6 @@ class CoupledConfig:
     k_ldf: float = 0.01                    # linear: kA; Langmuir: kLDF
     coupling: bool = True
     f_ref: float = 1.0                     # half storage: f = w * (f_ref + h)
-    g_ref: float = 0.0                     # half storage: g = w * (g_ref + h)
     meta: dict = dc_field(default_factory=dict)
 
     @property
@@ -82,7 +81,7 @@ class CoupledConfig:
             d.u_in[i] = float(self.u_in[i])
         d.rho_out, d.phi_in, d.tau_ad, d.min_conc = self.rho_out, self.phi_in, self.tau_ad, self.min_conc
         d.iso_a, d.iso_b, d.k_ldf = self.iso_a, self.iso_b, self.k_ldf
-        d.f_ref, d.g_ref = self.f_ref, self.g_ref
+        d.f_ref = self.f_ref
         return d
 
 
@@ -114,8 +113,9 @@ class CoupledSim:
         self.f = [torch.empty(19 * n, dtype=sdt, device=dev) for _ in range(2)]
         self.g = [torch.empty(7 * n, dtype=sdt, device=dev) for _ in range(2)]
         self.q = torch.zeros(n, dtype=self.cdt, device=dev)
-        self.phi = torch.zeros(n, dtype=self.cdt, device=dev)
+        self.phis = [torch.zeros(n, dtype=self.cdt, device=dev) for _ in range(2)]
         self.flags = torch.from_numpy(cfg.flags()).to(dev)
+        self.links = torch.empty(n, dtype=torch.int32, device=dev)
         self.field = None if cfg.field is None else torch.as_tensor(cfg.field, dtype=self.cdt).to(dev)
         self.tau_field = None if cfg.tau_field is None else torch.as_tensor(cfg.tau_field, dtype=self.cdt).to(dev)
         has_inlet = cfg.ns_inlet not in (None, "none")
@@ -130,7 +130,13 @@ class CoupledSim:
         self.step_index = 0
         self._acc = None
         self.stream = torch.cuda.current_stream(dev)
+        abi.call("fmx_coupled_build_links", self._desc_p(), self.flags.data_ptr(), self.links.data_ptr(), self._s())
         abi.call("fmx_coupled_init", self._desc_p(), self._bufs_p(), self.cur, float(cfg.phi0), self._s())
+
+    @property
+    def phi(self):
+        """Concentration of the current state (the last step's clamped pre-collision sum)."""
+        return self.phis[self.cur]
 
     # ------------------------------------------------------------ plumbing
     def _s(self):
@@ -143,7 +149,8 @@ class CoupledSim:
         b = abi.CoupledBufs()
         b.f[0], b.f[1] = self.f[0].data_ptr(), self.f[1].data_ptr()
         b.g[0], b.g[1] = self.g[0].data_ptr(), self.g[1].data_ptr()
-        b.q, b.phi, b.flags = self.q.data_ptr(), self.phi.data_ptr(), self.flags.data_ptr()
+        b.q, b.links = self.q.data_ptr(), self.links.data_ptr()
+        b.phi[0], b.phi[1] = self.phis[0].data_ptr(), self.phis[1].data_ptr()
         b.field = self.field.data_ptr() if self.field is not None else None
         b.tau_field = self.tau_field.data_ptr() if self.tau_field is not None else None
         b.inlet_raw = self.inlet_raw.data_ptr() if self.inlet_raw is not None else None

@@ -48,7 +48,7 @@ template <class C> struct CoupledParams {
     int block_x, block_y;
     int xcd_remap;
     i64 n;
-    C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref, g_ref;
+    C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
 };
 
 template <int P> struct CoupledPtrs {
@@ -59,8 +59,9 @@ template <int P> struct CoupledPtrs {
     const S *g;
     S *gn;
     C *q;
-    C *phi;
-    const std::uint8_t *flags;
+    const C *phi;   // previous step's concentration (sorption input, half-storage reference of g)
+    C *phi_new;     // this step's concentration
+    std::uint32_t *links;
     const C *field;
     const C *tau_field;
     C *inlet_raw;
@@ -88,6 +89,9 @@ __device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, 
 template <class T> __device__ __forceinline__ T *plane(T *base, int q, i64 n) { return base + i64(q) * n; }
 template <class T> __device__ __forceinline__ T ld_b(const T *base, unsigned boff) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T> __device__ __forceinline__ T ld_b_nt(const T *base, unsigned boff) {
+    return __builtin_nontemporal_load(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
 }
 template <class T> __device__ __forceinline__ void st_b(T *base, unsigned boff, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
@@ -118,7 +122,7 @@ template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy
     return w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - u2x15);
 }
 
-template <int P, int OP>
+template <int P, int OP, bool NTL>
 __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 acc_row) {
     using C = typename Storage<P>::C;
@@ -135,8 +139,11 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     const i64 n = prm.n;
     const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
     const unsigned cell_b = cell * SB;
-    const std::uint8_t fl = p.flags[cell];
-    const bool solid = (OP == FMX_NS_BGK) && (fl & FMX_FLAG_SOLID);
+    const std::uint32_t lk = p.links[cell];
+    const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
+    // a frozen solid cell alternates between the two states held in f and f_new and has zero
+    // post-collision velocity in both: no NS loads, collision or stores
+    const bool frozen = (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
 
     // neighbour rows: x keeps local out of domain, y/z wrap when periodic
     const bool xm_ok = x > 0, xp_ok = x < nx - 1;
@@ -165,21 +172,56 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
 
     // ---------------------------------------------------------------- NS pull
     C fo[19];
-    static_for<0, 19>([&](auto qc) {
+    if (!frozen) static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
         const unsigned src = src_of(cx, cy, cz);
-        bool bb = solid;
-        if constexpr (OP == FMX_NS_BGK && q != 0)
-            if (!solid && src != cell) bb = (p.flags[src] & FMX_FLAG_SOLID) != 0;
+        bool bb = false;
+        if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
         S raw;
-        if (bb) raw = ld_b(plane(p.f, Q19::opp[q], n), cell_b);
-        else raw = ld_b(plane(p.f, q, n), src * SB);
+        if constexpr (NTL) {
+            if (bb) raw = ld_b_nt(plane(p.f, Q19::opp[q], n), cell_b);
+            else raw = ld_b_nt(plane(p.f, q, n), src * SB);
+        } else {
+            if (bb) raw = ld_b(plane(p.f, Q19::opp[q], n), cell_b);
+            else raw = ld_b(plane(p.f, q, n), src * SB);
+        }
         fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
     });
 
+    // ------------------------------------------- AD pull and sorption inputs
+    // (issued with the NS pulls; nothing below writes what they read)
+    const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
+    C gi[7];
+    static_for<0, 7>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+        const unsigned s = src_of(cx, cy, cz);
+        // x of the source (or keep-local) cell: x - cx when in the domain, else x
+        const int sx = (cx == 1) ? (x > 0 ? x - 1 : x) : ((cx == -1) ? (x < nx - 1 ? x + 1 : x) : x);
+        if (prm.ad_inlet == 2 && sx == 0) {
+            gi[q] = phi_pre * w7<C>(q);
+        } else {
+            const S raw = NTL ? ld_b_nt(plane(p.g, q, n), s * SB) : ld_b(plane(p.g, q, n), s * SB);
+            if constexpr (IsHalf<P>::value) {
+                // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
+                // populations of its x - 1 neighbour, stored about that neighbour's concentration
+                const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
+                gi[q] = w7<C>(q) * (p.phi[sref] + C(raw));
+            } else {
+                gi[q] = raw;
+            }
+        }
+    });
+    const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
+    C phi_lag = C(0), q_old = C(0);
+    if (sorbs) {
+        phi_lag = p.phi[cell];
+        q_old = p.q[cell];
+    }
+
     // ----------------------------------------------------- NS collide (in place)
-    {
+    if (!frozen) {
         C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -239,16 +281,21 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     }
 
     // ------------------------------------- post-collision moments (Macros.cu:44-60)
-    C r2 = C(0), vx = C(0), vy = C(0), vz = C(0);
-    static_for<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        r2 += fo[q];
-        vx = axpy1<Q19::cx[q]>(vx, fo[q]);
-        vy = axpy1<Q19::cy[q]>(vy, fo[q]);
-        vz = axpy1<Q19::cz[q]>(vz, fo[q]);
-    });
-    const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
-    const C ux2 = vx * inv2, uy2 = vy * inv2, uz2 = vz * inv2;
+    C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
+    if (!frozen) {
+        C vx = C(0), vy = C(0), vz = C(0);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            r2 += fo[q];
+            vx = axpy1<Q19::cx[q]>(vx, fo[q]);
+            vy = axpy1<Q19::cy[q]>(vy, fo[q]);
+            vz = axpy1<Q19::cz[q]>(vz, fo[q]);
+        });
+        const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
+        ux2 = vx * inv2;
+        uy2 = vy * inv2;
+        uz2 = vz * inv2;
+    }
 
     // ------------------------------------------------------------- NS stores
     const bool inlet_cell = prm.ns_inlet && x == 0;
@@ -265,7 +312,7 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
                 st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
             });
         }
-    } else if (!outlet_cell) {
+    } else if (!outlet_cell && !frozen) {
 #pragma unroll
         for (int q = 0; q < 19; ++q)
             st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
@@ -294,8 +341,8 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
 
     // ------------------------------------------------------ sorption source
     C src_term = C(0);
-    if (prm.isotherm && (fl & FMX_FLAG_REGION)) {  // LinearLDF.cu:27-47, LangmuirLDF.cu:23-44
-        C c = p.phi[cell];
+    if (sorbs) {  // LinearLDF.cu:27-47, LangmuirLDF.cu:23-44
+        C c = phi_lag;
         if (c < prm.min_conc) c = prm.min_conc;
         const C ce = c - prm.min_conc;
         C qeq;
@@ -305,23 +352,11 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         } else {
             qeq = prm.iso_a * ce;
         }
-        const C qi = p.q[cell];
-        const C dq = prm.k_ldf * (qeq - qi);
-        p.q[cell] = qi + dq;
+        const C dq = prm.k_ldf * (qeq - q_old);
+        p.q[cell] = q_old + dq;
         src_term = -dq;
     }
 
-    // --------------------------------------------------------------- AD pull
-    const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
-    C gi[7];
-    static_for<0, 7>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-        const unsigned s = src_of(cx, cy, cz);
-        const bool on_inlet = (cx == 1) ? (x <= 1) : (x == 0);  // source (or keep-local) cell on x = 0
-        if (prm.ad_inlet == 2 && on_inlet) gi[q] = phi_pre * w7<C>(q);
-        else gi[q] = decode<P>(ld_b(plane(p.g, q, n), s * SB), w7<C>(q), prm.g_ref);
-    });
     C phi = C(0);
 #pragma unroll
     for (int q = 0; q < 7; ++q) phi += gi[q];
@@ -335,21 +370,22 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         const C geq = w7<C>(q) * phi * (C(1) + C(4) * eu);
         go[q] = gi[q] + om * (geq - gi[q]) + w7<C>(q) * src_term;
     });
-    p.phi[cell] = phi;
+    p.phi_new[cell] = phi;
 
     // ------------------------------------------------------------- AD stores
+    // (half storage: deviation about this cell's new concentration, which the readers load as phi)
     const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
     if (prm.ad_inlet == 1 && x == 0) {
 #pragma unroll
         for (int q = 0; q < 7; ++q)
-            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), prm.g_ref));
+            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
     } else if (!ad_out_cell) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), prm.g_ref));
+        for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
     }
     if (prm.ad_outlet && x == nx - 2) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), prm.g_ref));
+        for (int q = 0; q < 7; ++q) st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), phi));
     }
 
     // ------------------------------------------------- outlet breakthrough sum
@@ -359,6 +395,42 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
                         double(phi));
     }
     (void)CB;
+}
+
+// Link words: bounce-back bits of the D3Q19 pulls (BGKCollision.cu:90-97, 124-126), region, solid.
+__global__ void __launch_bounds__(256) k_build_links(const std::uint8_t *__restrict__ flags,
+                                                     std::uint32_t *__restrict__ links, int nx, int ny, int nz,
+                                                     int py, int pz, int bgk) {
+    const i64 c = gtid();
+    const i64 n = i64(nx) * ny * nz;
+    if (c >= n) return;
+    const int x = int(c % nx);
+    const int y = int((c / nx) % ny);
+    const int z = int(c / (i64(nx) * ny));
+    const std::uint8_t f = flags[c];
+    std::uint32_t w = 0;
+    if (f & FMX_FLAG_REGION) w |= FMX_LINK_REGION;
+    if (f & FMX_FLAG_SOLID) w |= FMX_LINK_SOLID;
+    if (bgk) {
+        for (int q = 1; q < 19; ++q) {
+            if (f & FMX_FLAG_SOLID) {
+                w |= 1u << q;
+                continue;
+            }
+            int xs = x - Q19::cx[q], ys = y - Q19::cy[q], zs = z - Q19::cz[q];
+            if (xs < 0 || xs >= nx) continue;  // keep local
+            if (ys < 0 || ys >= ny) {
+                if (!py) continue;
+                ys = (ys + ny) % ny;
+            }
+            if (zs < 0 || zs >= nz) {
+                if (!pz) continue;
+                zs = (zs + nz) % nz;
+            }
+            if (flags[cell_index(xs, ys, zs, nx, ny)] & FMX_FLAG_SOLID) w |= 1u << q;
+        }
+    }
+    links[c] = w;
 }
 
 // First step's inlet update (NEQ on f(0,y,z) from f(1,y,z)) and the raw x = 0 plane.
@@ -410,21 +482,71 @@ __global__ void __launch_bounds__(256) k_coupled_prologue(typename Storage<P>::S
     }
 }
 
-// Reference-view read-back: decode populations; the x = 0 plane comes from inlet_raw.
+// Frozen-solid set-up: a BGK solid cell away from the inlet/outlet update planes only reverses its
+// own populations (BGKCollision.cu:90-97), so it alternates between its state A and A[opp].  When
+// both phases give exactly zero post-collision velocity in the kernel's arithmetic, the cell is
+// marked frozen and its next phase is written to the other buffer once.
+template <int P>
+__global__ void __launch_bounds__(256) k_freeze(const typename Storage<P>::S *__restrict__ fcur,
+                                                typename Storage<P>::S *__restrict__ fnext,
+                                                std::uint32_t *__restrict__ links,
+                                                CoupledParams<typename Storage<P>::C> prm, int allow) {
+    using C = typename Storage<P>::C;
+    const i64 c = gtid();
+    const i64 n = prm.n;
+    if (c >= n) return;
+    const int x = int(c % prm.nx);
+    std::uint32_t lk = links[c] & ~FMX_LINK_FROZEN;
+    if (allow && (lk & FMX_LINK_SOLID) && x >= 2 && x <= prm.nx - 3) {
+        C a[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) a[q] = ld_pop<P>(fcur, i64(q) * n + c, w19<C>(q), prm.f_ref);
+        bool still = true;
+        for (int phase = 0; phase < 2; ++phase) {  // phase 0: post-collision = a[opp]; phase 1: = a
+            C r = C(0), vx = C(0), vy = C(0), vz = C(0);
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C v = phase == 0 ? a[Q19::opp[q]] : a[q];
+                r += v;
+                vx = axpy1<Q19::cx[q]>(vx, v);
+                vy = axpy1<Q19::cy[q]>(vy, v);
+                vz = axpy1<Q19::cz[q]>(vz, v);
+            });
+            const C inv = (r > C(0)) ? (C(1) / r) : C(0);
+            still = still && vx == C(0) && vy == C(0) && vz == C(0) && inv - inv == C(0);
+        }
+        if (still) {
+            lk |= FMX_LINK_FROZEN;
+#pragma unroll
+            for (int q = 0; q < 19; ++q) st_pop<P>(fnext, i64(q) * n + c, a[Q19::opp[q]], C(Q19::inv_w[q]), prm.f_ref);
+        }
+    }
+    links[c] = lk;
+}
+
+// Reference-view read-back: decode populations; the x = 0 plane comes from inlet_raw.  Half
+// storage of g is relative to the lagged concentration (the x - 1 cell's on a Neumann outlet).
 template <int P, int Q>
 __global__ void __launch_bounds__(256) k_coupled_view(const typename Storage<P>::S *pop,
                                                       const typename Storage<P>::C *inlet_raw,
-                                                      typename Storage<P>::C *out, int nx, i64 np, i64 n,
-                                                      typename Storage<P>::C ref) {
+                                                      const typename Storage<P>::C *phi, typename Storage<P>::C *out,
+                                                      int nx, i64 np, i64 n, typename Storage<P>::C ref, int ad_outlet) {
     using C = typename Storage<P>::C;
     const i64 c = gtid();
     if (c >= n) return;
     const i64 xx = c % nx;
+    const i64 sref = (ad_outlet && xx == nx - 1) ? c - 1 : c;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const C w = (Q == 19) ? w19<C>(q) : w7<C>(q);
-        C v = ld_pop<P>(pop, i64(q) * n + c, w, ref);
-        if (inlet_raw && xx == 0) v = inlet_raw[q * np + c / nx];
+        C v;
+        if constexpr (Q == 19) {
+            v = ld_pop<P>(pop, i64(q) * n + c, w19<C>(q), ref);
+            if (inlet_raw && xx == 0) v = inlet_raw[q * np + c / nx];
+        } else if constexpr (IsHalf<P>::value) {
+            v = w7<C>(q) * (phi[sref] + C(pop[i64(q) * n + c]));
+        } else {
+            v = pop[i64(q) * n + c];
+        }
         out[q * n + c] = v;
     }
 }
@@ -432,16 +554,21 @@ __global__ void __launch_bounds__(256) k_coupled_view(const typename Storage<P>:
 // Encode populations into storage: dst[q*n + c] = src ? src[q*n + c] : w_q * value.
 template <int P, int Q>
 __global__ void __launch_bounds__(256) k_encode_pop(typename Storage<P>::S *dst, const typename Storage<P>::C *src,
-                                                    i64 n, typename Storage<P>::C value, typename Storage<P>::C ref) {
+                                                    i64 n, typename Storage<P>::C value, typename Storage<P>::C ref,
+                                                    const typename Storage<P>::C *phi, int nx, int ad_outlet) {
     using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
     const i64 c = gtid();
     if (c >= n) return;
+    const i64 xx = c % nx;
+    const i64 sref = (ad_outlet && xx == nx - 1) ? c - 1 : c;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const C w = (Q == 19) ? w19<C>(q) : w7<C>(q);
         const int iw = (Q == 19) ? Q19::inv_w[q] : Q7::inv_w[q];
         const C v = src ? src[q * n + c] : w * value;
-        st_pop<P>(dst, i64(q) * n + c, v, C(iw), ref);
+        if constexpr (Q == 7 && IsHalf<P>::value) dst[i64(q) * n + c] = S(v * C(iw) - phi[sref]);
+        else st_pop<P>(dst, i64(q) * n + c, v, C(iw), ref);
     }
 }
 
@@ -463,7 +590,7 @@ __global__ void __launch_bounds__(256) k_outlet_mean(const double *__restrict__ 
 
 // ------------------------------------------------------------- host side
 struct LaunchShape {
-    int bx = 64, by = 4, remap = 1;
+    int bx = 256, by = 1, remap = 1;
 };
 
 static LaunchShape launch_shape() {
@@ -516,7 +643,6 @@ template <class C> static CoupledParams<C> make_params(const fmx_coupled_desc &d
     p.iso_b = C(d.iso_b);
     p.k_ldf = C(d.k_ldf);
     p.f_ref = C(d.f_ref);
-    p.g_ref = C(d.g_ref);
     return p;
 }
 
@@ -529,8 +655,9 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     p.g = static_cast<const S *>(b.g[cur]);
     p.gn = static_cast<S *>(b.g[1 - cur]);
     p.q = static_cast<C *>(b.q);
-    p.phi = static_cast<C *>(b.phi);
-    p.flags = b.flags;
+    p.phi = static_cast<const C *>(b.phi[cur]);
+    p.phi_new = static_cast<C *>(b.phi[1 - cur]);
+    p.links = b.links;
     p.field = static_cast<const C *>(b.field);
     p.tau_field = static_cast<const C *>(b.tau_field);
     p.inlet_raw = static_cast<C *>(b.inlet_raw);
@@ -546,9 +673,15 @@ static int run_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 s
     const CoupledParams<C> prm = make_params<C>(d);
     const unsigned nblocks = unsigned(prm.gx) * unsigned(prm.gy) * unsigned(prm.gz);
     const dim3 block(prm.block_x, prm.block_y, 1);
+    // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads
+    const char *ntl = std::getenv("FMX_NT_LOADS");
+    const bool nt_loads = ntl != nullptr && std::atoi(ntl) != 0;
     for (int k = 0; k < nsteps; ++k) {
         const CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
-        hipLaunchKernelGGL((k_coupled<P, OP>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
+        if (nt_loads)
+            hipLaunchKernelGGL((k_coupled<P, OP, true>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
+        else
+            hipLaunchKernelGGL((k_coupled<P, OP, false>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
         const int rc = after_launch("k_coupled");
         if (rc) return rc;
     }
@@ -571,10 +704,21 @@ static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
     const CoupledParams<C> prm = make_params<C>(d);
-    const i64 np = i64(d.ny) * d.nz;
-    hipLaunchKernelGGL(k_coupled_prologue<P>, dim3(blocks_for(np)), dim3(256), 0, s, static_cast<S *>(b.f[cur]),
-                       static_cast<C *>(b.inlet_raw), prm);
-    return after_launch("k_coupled_prologue");
+    if (b.inlet_raw) {
+        const i64 np = i64(d.ny) * d.nz;
+        hipLaunchKernelGGL(k_coupled_prologue<P>, dim3(blocks_for(np)), dim3(256), 0, s, static_cast<S *>(b.f[cur]),
+                           static_cast<C *>(b.inlet_raw), prm);
+        if (int rc = after_launch("k_coupled_prologue")) return rc;
+    }
+    if (d.ns_op != FMX_NS_BGK) return FMX_OK;
+    // Opt-in (FMX_FREEZE=1): with the structure-of-arrays layout a frozen cell only saves traffic
+    // when whole cache lines are frozen, which dense sphere packs rarely give (measured slower on
+    // the 256x128x128 pack), so by default every solid cell stays on the full path.
+    const char *fz = std::getenv("FMX_FREEZE");
+    const int allow = fz != nullptr && std::atoi(fz) != 0;
+    hipLaunchKernelGGL(k_freeze<P>, dim3(blocks_for(prm.n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
+                       static_cast<S *>(b.f[1 - cur]), b.links, prm, allow);
+    return after_launch("k_freeze");
 }
 
 template <int P>
@@ -584,39 +728,40 @@ static int load_state(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int 
     using S = typename Storage<P>::S;
     const i64 n = i64(d.nx) * d.ny * d.nz;
     const unsigned nb = blocks_for(n);
+    C *phi = static_cast<C *>(b.phi[cur]);
+    if (init || phi_c) {  // first: g is stored relative to phi in half precision
+        hipLaunchKernelGGL(k_copy_or_zero<C>, dim3(nb), dim3(256), 0, s, phi, static_cast<const C *>(phi_c), n);
+        if (int rc = after_launch("k_copy_or_zero")) return rc;
+    }
     if (init || f_c) {
         hipLaunchKernelGGL((k_encode_pop<P, 19>), dim3(nb), dim3(256), 0, s, static_cast<S *>(b.f[cur]),
-                           static_cast<const C *>(f_c), n, C(d.rho0), C(d.f_ref));
+                           static_cast<const C *>(f_c), n, C(d.rho0), C(d.f_ref), phi, d.nx, d.ad_outlet);
         if (int rc = after_launch("k_encode_pop")) return rc;
     }
     if (init || g_c) {
         hipLaunchKernelGGL((k_encode_pop<P, 7>), dim3(nb), dim3(256), 0, s, static_cast<S *>(b.g[cur]),
-                           static_cast<const C *>(g_c), n, C(phi0), C(d.g_ref));
+                           static_cast<const C *>(g_c), n, C(phi0), C(0), phi, d.nx, d.ad_outlet);
         if (int rc = after_launch("k_encode_pop")) return rc;
     }
     if (init || q_c) {
         hipLaunchKernelGGL(k_copy_or_zero<C>, dim3(nb), dim3(256), 0, s, static_cast<C *>(b.q), static_cast<const C *>(q_c), n);
         if (int rc = after_launch("k_copy_or_zero")) return rc;
     }
-    if (init || phi_c) {
-        hipLaunchKernelGGL(k_copy_or_zero<C>, dim3(nb), dim3(256), 0, s, static_cast<C *>(b.phi),
-                           static_cast<const C *>(phi_c), n);
-        if (int rc = after_launch("k_copy_or_zero")) return rc;
-    }
-    if (b.inlet_raw) return prologue<P>(d, b, cur, s);
-    return FMX_OK;
+    return prologue<P>(d, b, cur, s);
 }
 
 template <int P, int Q>
-static int view(const fmx_coupled_desc &d, const void *pop, const void *inlet_raw, void *out, double ref,
-                hipStream_t s) {
+static int view(const fmx_coupled_desc &d, const void *pop, const void *inlet_raw, const void *phi, void *out,
+                double ref, hipStream_t s) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
     const i64 n = i64(d.nx) * d.ny * d.nz;
     hipLaunchKernelGGL((k_coupled_view<P, Q>), dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const S *>(pop),
-                       static_cast<const C *>(inlet_raw), static_cast<C *>(out), d.nx, i64(d.ny) * d.nz, n, C(ref));
+                       static_cast<const C *>(inlet_raw), static_cast<const C *>(phi), static_cast<C *>(out), d.nx,
+                       i64(d.ny) * d.nz, n, C(ref), d.ad_outlet);
     return after_launch("k_coupled_view");
 }
+
 
 }  // namespace fmx
 
@@ -643,7 +788,7 @@ int fmx_coupled_validate(const fmx_coupled_desc *d, const fmx_coupled_bufs *b) {
                          "fmx_coupled: a pressure outlet needs the AD Neumann outlet in the fused step (use the "
                          "operator API for other combinations)");
     if (!(d->tau_ad > 0.0) && !b->tau_field) return set_error(FMX_ERR_ARG, "fmx_coupled: tau_ad must be > 0");
-    if (!b->f[0] || !b->f[1] || !b->g[0] || !b->g[1] || !b->q || !b->phi || !b->flags)
+    if (!b->f[0] || !b->f[1] || !b->g[0] || !b->g[1] || !b->q || !b->phi[0] || !b->phi[1] || !b->links)
         return set_error(FMX_ERR_ARG, "fmx_coupled: missing state buffer");
     if (d->ns_inlet && !b->inlet_raw) return set_error(FMX_ERR_ARG, "fmx_coupled: ns_inlet needs inlet_raw");
     if (d->ad_inlet == 2 && !b->inlet_series) return set_error(FMX_ERR_ARG, "fmx_coupled: ad_inlet 2 needs inlet_series");
@@ -652,12 +797,23 @@ int fmx_coupled_validate(const fmx_coupled_desc *d, const fmx_coupled_bufs *b) {
     return FMX_OK;
 }
 
+int fmx_coupled_build_links(const fmx_coupled_desc *d, const std::uint8_t *flags, std::uint32_t *links,
+                            void *stream) {
+    FMX_TRY_BEGIN
+    if (!d || !flags || !links || d->nx < 1 || d->ny < 1 || d->nz < 1)
+        return set_error(FMX_ERR_ARG, "fmx_coupled_build_links: bad arguments");
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    hipLaunchKernelGGL(k_build_links, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), flags, links, d->nx, d->ny,
+                       d->nz, d->periodic_y != 0, d->periodic_z != 0, d->ns_op == FMX_NS_BGK);
+    return after_launch("k_build_links");
+    FMX_TRY_END
+}
+
 int fmx_coupled_prologue(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, void *stream) {
     FMX_TRY_BEGIN
     int rc = fmx_coupled_validate(d, b);
     if (rc) return rc;
     if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_prologue: cur must be 0 or 1");
-    if (!b->inlet_raw) return FMX_OK;
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
         case FMX_F32: return prologue<FMX_F32>(*d, *b, cur, s);
@@ -724,10 +880,10 @@ int fmx_coupled_view_f(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int
     const void *raw = d->ns_inlet ? b->inlet_raw : nullptr;
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
-        case FMX_F32: return view<FMX_F32, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
-        case FMX_F64: return view<FMX_F64, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
-        case FMX_F16: return view<FMX_F16, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
-        default: return view<FMX_BF16, 19>(*d, b->f[cur], raw, f_out, d->f_ref, s);
+        case FMX_F32: return view<FMX_F32, 19>(*d, b->f[cur], raw, nullptr, f_out, d->f_ref, s);
+        case FMX_F64: return view<FMX_F64, 19>(*d, b->f[cur], raw, nullptr, f_out, d->f_ref, s);
+        case FMX_F16: return view<FMX_F16, 19>(*d, b->f[cur], raw, nullptr, f_out, d->f_ref, s);
+        default: return view<FMX_BF16, 19>(*d, b->f[cur], raw, nullptr, f_out, d->f_ref, s);
     }
     FMX_TRY_END
 }
@@ -739,10 +895,10 @@ int fmx_coupled_view_g(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int
     if ((cur != 0 && cur != 1) || !g_out) return set_error(FMX_ERR_ARG, "fmx_coupled_view_g: bad arguments");
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
-        case FMX_F32: return view<FMX_F32, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
-        case FMX_F64: return view<FMX_F64, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
-        case FMX_F16: return view<FMX_F16, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
-        default: return view<FMX_BF16, 7>(*d, b->g[cur], nullptr, g_out, d->g_ref, s);
+        case FMX_F32: return view<FMX_F32, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
+        case FMX_F64: return view<FMX_F64, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
+        case FMX_F16: return view<FMX_F16, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
+        default: return view<FMX_BF16, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
     }
     FMX_TRY_END
 }

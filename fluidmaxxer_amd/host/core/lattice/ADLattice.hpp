@@ -119,6 +119,11 @@ public:
     }
 
     void swapPopulations() { std::swap(d_g, d_g_new); }
+    // replace the concentration buffer (fused steppers ping-pong it); returns the previous one
+    DATA_TYPE *exchangePhi(DATA_TYPE *p) {
+        std::swap(d_phi, p);
+        return p;
+    }
 
 private:
     LBMParams params{};

@@ -63,7 +63,9 @@ public:
     }
     ~CoupledStep() {
         fmx::freeBytes(d_flags_);
+        fmx::freeBytes(d_links_);
         fmx::freeBytes(d_inlet_raw_);
+        fmx::freeBytes(d_phi_alt_);
         fmx::freeBytes(d_series_);
         fmx::freeBytes(d_acc_);
         fmx::freeBytes(d_mean_);
@@ -98,6 +100,7 @@ public:
         if (n & 1) {
             ns_.swapPopulations();
             ad_.swapPopulations();
+            d_phi_alt_ = ad_.exchangePhi(d_phi_alt_);
         }
         if (logOutlet) {
             fmx::check(fmx_outlet_mean(d_acc_, n, std::int64_t(nyz_), d_mean_, nullptr), "fmx_outlet_mean");
@@ -135,8 +138,11 @@ private:
                                     (!region_.empty() && region_[i] ? FMX_FLAG_REGION : 0));
         d_flags_ = static_cast<std::uint8_t *>(fmx::mallocBytes(flags.size()));
         fmx::memcpyHtoD(d_flags_, flags.data(), flags.size());
+        d_links_ = static_cast<std::uint32_t *>(fmx::mallocBytes(flags.size() * sizeof(std::uint32_t)));
+        fmx::check(fmx_coupled_build_links(&desc_, d_flags_, d_links_, nullptr), "fmx_coupled_build_links");
         if (desc_.ns_inlet)
             d_inlet_raw_ = static_cast<DATA_TYPE *>(fmx::mallocBytes(sizeof(DATA_TYPE) * 19 * nyz_));
+        d_phi_alt_ = static_cast<DATA_TYPE *>(fmx::mallocBytes(sizeof(DATA_TYPE) * std::size_t(ns_.size64())));
     }
 
     void analyse() {
@@ -240,8 +246,9 @@ private:
         bufs_.g[0] = ad_.d_g_ptr();
         bufs_.g[1] = ad_.d_g_new_ptr();
         bufs_.q = q_.d_q();
-        bufs_.phi = ad_.d_phi_ptr();
-        bufs_.flags = d_flags_;
+        bufs_.phi[0] = ad_.d_phi_ptr();
+        bufs_.phi[1] = d_phi_alt_;
+        bufs_.links = d_links_;
         bufs_.field = desc_.ns_op == FMX_NS_BGK ? nullptr : ns_.d_porosity_ptr();
         bufs_.tau_field = tau_field_;
         bufs_.inlet_raw = d_inlet_raw_;
@@ -322,7 +329,9 @@ private:
     fmx_coupled_desc desc_{};
     fmx_coupled_bufs bufs_{};
     std::uint8_t *d_flags_ = nullptr;
+    std::uint32_t *d_links_ = nullptr;
     DATA_TYPE *d_inlet_raw_ = nullptr;
+    DATA_TYPE *d_phi_alt_ = nullptr;
     const DATA_TYPE *tau_field_ = nullptr;
     std::function<double(int)> series_;
     double *d_series_ = nullptr;

@@ -38,6 +38,15 @@ enum { FMX_ISO_NONE = 0, FMX_ISO_LINEAR = 1, FMX_ISO_LANGMUIR = 2 };
 enum { FMX_OK = 0, FMX_ERR_ARG = 1, FMX_ERR_HIP = 2, FMX_ERR_UNSUPPORTED = 3 };
 /* flag byte bits of the coupled step */
 enum { FMX_FLAG_SOLID = 1, FMX_FLAG_REGION = 2 };
+/* per-cell link word of the coupled step (fmx_coupled_build_links): bit q (1..18) set when the
+ * D3Q19 pull along direction q bounces back (the cell is solid, or the in-domain source cell is
+ * solid; FMX_NS_BGK only), plus the region and solid bits */
+#define FMX_LINK_REGION (1u << 19)
+#define FMX_LINK_SOLID (1u << 20)
+/* set by fmx_coupled_prologue on FMX_NS_BGK solid cells (2 <= x <= nx-3) whose populations and their
+ * reversal both carry exactly zero momentum: such a cell alternates between two fixed states, both are
+ * kept in the two population buffers, and the step skips its NS loads and stores */
+#define FMX_LINK_FROZEN (1u << 21)
 
 typedef struct fmx_grid {
     int32_t nx, ny, nz;
@@ -155,7 +164,9 @@ typedef struct fmx_coupled_desc {
     double tau_ad;                  /* AD relaxation time when tau_field is NULL */
     double min_conc;                /* LBMParams::minConcentration */
     double iso_a, iso_b, k_ldf;     /* linear: a = KHenry, k = kA; Langmuir: a = qMax, b = bLangmuir, k = kLDF */
-    double f_ref, g_ref;            /* half storage: p = w * (ref + h) */
+    double f_ref;                   /* half storage of f: f = w * (f_ref + h); half storage of g holds
+                                       g = w * (phi_src + h), phi_src the source cell's lagged
+                                       concentration (the x = nx-2 value on a Neumann outlet plane) */
 } fmx_coupled_desc;
 
 #define FMX_OUTLET_SLOTS 64
@@ -165,8 +176,10 @@ typedef struct fmx_coupled_bufs {
     void *f[2];               /* 19*N populations, ping-pong */
     void *g[2];               /* 7*N populations, ping-pong */
     void *q;                  /* N adsorbed load (AdsorbedField) */
-    void *phi;                /* N concentration: clamped pre-collision sum, the next step's sorption input */
-    const uint8_t *flags;     /* N: FMX_FLAG_SOLID | FMX_FLAG_REGION */
+    void *phi[2];             /* N concentration, ping-pong with f/g: phi[cur] is the previous step's clamped
+                                 pre-collision sum (sorption input), the step writes phi[1-cur] */
+    uint32_t *links;          /* N link words from fmx_coupled_build_links (the prologue updates the
+                                 FMX_LINK_FROZEN bits) */
     const void *field;        /* N porosity (POROUS) or solid fraction (PBB); NULL = 1 / 0 */
     const void *tau_field;    /* N AD relaxation times; NULL = tau_ad */
     void *inlet_raw;          /* 19*ny*nz, required with ns_inlet != 0: the x = 0 plane's populations as the
@@ -178,19 +191,23 @@ typedef struct fmx_coupled_bufs {
 } fmx_coupled_bufs;
 
 int fmx_coupled_validate(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs);
+/* Builds the per-cell link words from the flag bytes (FMX_FLAG_SOLID | FMX_FLAG_REGION per cell)
+ * for the geometry and periodicity of `desc`; run once per geometry */
+int fmx_coupled_build_links(const fmx_coupled_desc *desc, const uint8_t *flags, uint32_t *links, void *stream);
 /* Lattice::initEquilibrium + ADLattice::initEquilibrium (NSLattice.hpp:177-185, ADLattice.hpp:134-141,
  * BGKCollision.cu:44-71, ADBGKCollision.cu:29-38) for the fused state: f[cur] = w*rho0, g[cur] = w*phi0,
- * q = 0, phi = 0 (AdsorbedField.hpp:19-20, ADLattice.hpp:49), followed by fmx_coupled_prologue */
+ * q = 0, phi[cur] = 0 (AdsorbedField.hpp:19-20, ADLattice.hpp:49), followed by fmx_coupled_prologue */
 int fmx_coupled_init(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, double phi0, void *stream);
 /* Loads a state given in compute precision SoA (f_c 19*N, g_c 7*N, q_c N, phi_c N; each nullable = keep)
- * into f[cur], g[cur], q, phi, followed by fmx_coupled_prologue */
+ * into f[cur], g[cur], q, phi[cur], followed by fmx_coupled_prologue */
 int fmx_coupled_load(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, const void *f_c,
                      const void *g_c, const void *q_c, const void *phi_c, void *stream);
-/* Copies the x = 0 plane of f[cur] to inlet_raw and applies the first step's inlet update to f[cur]
- * (call once after initialisation, before the first fmx_coupled_run) */
+/* Copies the x = 0 plane of f[cur] to inlet_raw, applies the first step's inlet update to f[cur] and
+ * marks frozen solid cells in bufs->links, writing their next phase into f[1-cur] (call after any
+ * change of state made outside fmx_coupled_run, before the next fmx_coupled_run) */
 int fmx_coupled_prologue(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *stream);
-/* Steps step0 .. step0+nsteps-1.  State is read from f[cur], g[cur]; step k writes buffer
- * (cur + k + 1) % 2, so the result is in (cur + nsteps) % 2. */
+/* Steps step0 .. step0+nsteps-1.  State is read from f[cur], g[cur], phi[cur]; step k writes
+ * buffer (cur + k + 1) % 2, so the result is in (cur + nsteps) % 2. */
 int fmx_coupled_run(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
                     int cur, void *stream);
 /* The NS populations of f[cur] as the reference holds them (x = 0 plane from inlet_raw), converted to
@@ -208,6 +225,10 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  * f is SoA in `prec` storage (half storage needs f_ref); any input may be NULL (sum = 0) */
 int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
                     void *stream);
+
+/* Streaming-bandwidth probes (16 B per lane, non-temporal): mode 0 copies `bytes` from src to dst,
+ * mode 1 only reads src (dst: a 4-byte sink).  bytes must be a multiple of 16. */
+int fmx_stream_probe(int mode, void *dst, const void *src, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------ geometry */
 /* Seeded sphere pack (SplitMix64 -> xoshiro256**): solid walls on y in {0, ny-1} and z in {0, nz-1},

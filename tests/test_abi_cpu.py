@@ -30,7 +30,8 @@ def test_validate_rejects_bad_descriptors():
     b = abi.CoupledBufs()
     for i in range(2):
         b.f[i] = b.g[i] = 0x1000
-    b.q = b.phi = b.flags = b.inlet_raw = 0x1000
+    b.q = b.links = b.inlet_raw = 0x1000
+    b.phi[0] = b.phi[1] = 0x1000
     assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 0
     d.nx = 2
     assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) != 0
