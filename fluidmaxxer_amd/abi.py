@@ -21,6 +21,7 @@ NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE = 64, 16
+ADJ_SLOTS, ADJ_STRIDE = 64, 16
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
 
@@ -56,6 +57,15 @@ class CoupledBufs(C.Structure):
         ("q", C.c_void_p), ("phi", C.c_void_p * 2),
         ("links", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
         ("inlet_raw", C.c_void_p), ("inlet_series", C.c_void_p), ("outlet_acc", C.c_void_p),
+    ]
+
+
+class AdjointBufs(C.Structure):
+    _fields_ = [
+        ("lam_f", C.c_void_p * 2), ("lam_g", C.c_void_p * 2),
+        ("lam_q", C.c_void_p), ("lam_phi", C.c_void_p),
+        ("iso_acc", C.c_void_p), ("outlet_weight", C.c_void_p),
+        ("grad_iso", C.c_void_p), ("grad_field", C.c_void_p), ("grad_series", C.c_void_p),
     ]
 
 
@@ -108,6 +118,9 @@ _SIGS = {
     "fmx_coupled_view_f": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_coupled_view_g": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_outlet_mean": ([_vp, _i64, _i64, _vp, _vp], _i),
+    "fmx_adjoint_state_bytes": ([_DP], C.c_int64),
+    "fmx_adjoint_forward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _i, _vp, _vp], _i),
+    "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _vp, C.POINTER(AdjointBufs), _vp], _i),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
     "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,

@@ -1,0 +1,183 @@
+"""Breakthrough-curve gradients through the discrete adjoint of the fused coupled step.
+
+`AdjointRun` owns one coupled configuration on the GPU, a checkpoint store and the
+adjoint buffers.  `forward()` runs the configuration from its initial state for
+`nsteps` steps (fmx_adjoint_forward, a checkpoint every `k` steps) and returns the
+outlet-averaged concentration C_out(s), s = 1..nsteps.  `backward(w)` runs the
+reverse sweep (fmx_adjoint_backward) for a loss with dL/dC_out(s) = w[s-1] and
+returns the gradients with respect to the isotherm constants (Langmuir q_max and
+K_L, or the linear K_Henry), the per-cell field (partial-bounce-back solid fraction
+or porosity) and the per-step inlet concentration series.
+
+`CoupledBreakthrough` wraps the pair as a torch.autograd.Function:
+
+    run = AdjointRun(cfg, nsteps=4000, k=200, prec="f32")
+    c = CoupledBreakthrough.apply(run, q_max, k_l, field, series)   # tensors
+    loss = ((c - target) ** 2).sum(); loss.backward()
+
+Both passes run entirely on the GPU through the C ABI; nothing here computes a
+population update.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .coupled import CoupledConfig, CoupledSim
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class AdjointRun:
+    def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0"):
+        torch = _torch()
+        if prec not in ("f32", "f64"):
+            raise abi.FmxError("the adjoint runs on f32 or f64 state")
+        if cfg.ns_inlet == "neq":
+            raise abi.FmxError("the NEQ velocity inlet has no adjoint; use ns_inlet='eq'")
+        if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):
+            raise abi.FmxError("inlet_series must hold steps 0..nsteps")
+        self.cfg, self.nsteps, self.k = cfg, int(nsteps), int(k)
+        self.sim = CoupledSim(cfg, prec=prec, device=device)
+        self.device = self.sim.device
+        self.cdt = self.sim.cdt
+        n = cfg.n
+        d = self.sim._desc
+        self.state_bytes = int(abi.lib().fmx_adjoint_state_bytes(abi.C.byref(d)))
+        nseg = (self.nsteps + self.k - 1) // self.k
+        u8 = torch.uint8
+        self.ckpt = torch.empty(nseg * self.state_bytes, dtype=u8, device=self.device)
+        self.seg = torch.empty(max(self.k - 1, 1) * self.state_bytes, dtype=u8, device=self.device)
+        self.lam_f = [torch.empty(19 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
+        self.lam_g = [torch.empty(7 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
+        self.lam_q = torch.empty(n, dtype=self.cdt, device=self.device)
+        self.lam_phi = torch.empty(n, dtype=self.cdt, device=self.device)
+        self.iso_acc = torch.empty(abi.ADJ_SLOTS * abi.ADJ_STRIDE, dtype=torch.float64, device=self.device)
+        self.grad_iso = torch.zeros(2, dtype=torch.float64, device=self.device)
+        self.grad_field = torch.zeros(n, dtype=torch.float64, device=self.device) if self.sim.field is not None else None
+        self.grad_series = (torch.zeros(self.sim.inlet_series.numel(), dtype=torch.float64, device=self.device)
+                            if self.sim.inlet_series is not None else None)
+        self.acc = None
+        self.forward_done = False
+
+    # ------------------------------------------------------------ parameters
+    def set_params(self, iso_a=None, iso_b=None, field=None, series=None):
+        """Update the differentiable inputs in place (device tensors or arrays)."""
+        torch = _torch()
+        d = self.sim._desc
+        if iso_a is not None:
+            d.iso_a = float(iso_a)
+        if iso_b is not None:
+            d.iso_b = float(iso_b)
+        if field is not None:
+            if self.sim.field is None:
+                raise abi.FmxError("this configuration has no field")
+            self.sim.field.copy_(torch.as_tensor(field).to(self.device, self.cdt).reshape(-1))
+        if series is not None:
+            if self.sim.inlet_series is None:
+                raise abi.FmxError("this configuration has no inlet series")
+            self.sim.inlet_series.copy_(torch.as_tensor(series).to(self.device, torch.float64).reshape(-1))
+        self.forward_done = False
+
+    # ------------------------------------------------------------ passes
+    def forward(self):
+        """C_out(s) for s = 1..nsteps (float64 device tensor), from the configuration's initial state."""
+        torch = _torch()
+        sim = self.sim
+        sim.cur, sim.step_index = 0, 0
+        abi.call("fmx_coupled_init", sim._desc_p(), sim._bufs_p(), 0, float(self.cfg.phi0), sim._s())
+        self.acc = sim.outlet_acc(self.nsteps)
+        abi.call("fmx_adjoint_forward", sim._desc_p(), sim._bufs_p(self.acc), 1, self.nsteps, self.k, 0,
+                 self.ckpt.data_ptr(), sim._s())
+        sim.cur = self.nsteps & 1
+        sim.step_index = self.nsteps
+        mean = torch.empty(self.nsteps, dtype=torch.float64, device=self.device)
+        abi.call("fmx_outlet_mean", self.acc.data_ptr(), self.nsteps, self.cfg.ny * self.cfg.nz, mean.data_ptr(),
+                 sim._s())
+        self.forward_done = True
+        return mean
+
+    def backward(self, weight):
+        """Gradients of a loss with dL/dC_out(s) = weight[s-1]: dict of iso_a, iso_b (floats),
+        field (per cell, or None) and series (indexed like inlet_series, or None)."""
+        torch = _torch()
+        if not self.forward_done:
+            raise abi.FmxError("backward() needs a forward() with the current parameters")
+        w = torch.as_tensor(weight).to(self.device, torch.float64).contiguous()
+        if w.numel() != self.nsteps:
+            raise abi.FmxError(f"weight must have {self.nsteps} entries")
+        a = abi.AdjointBufs()
+        a.lam_f[0], a.lam_f[1] = self.lam_f[0].data_ptr(), self.lam_f[1].data_ptr()
+        a.lam_g[0], a.lam_g[1] = self.lam_g[0].data_ptr(), self.lam_g[1].data_ptr()
+        a.lam_q, a.lam_phi = self.lam_q.data_ptr(), self.lam_phi.data_ptr()
+        a.iso_acc = self.iso_acc.data_ptr()
+        a.outlet_weight = w.data_ptr()
+        a.grad_iso = self.grad_iso.data_ptr()
+        a.grad_field = self.grad_field.data_ptr() if self.grad_field is not None else None
+        a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
+        sim = self.sim
+        abi.call("fmx_adjoint_backward", sim._desc_p(), sim._bufs_p(), 1, self.nsteps, self.k, self.ckpt.data_ptr(),
+                 self.seg.data_ptr(), abi.C.byref(a), sim._s())
+        sim.synchronize()
+        gi = self.grad_iso.cpu().numpy()
+        return {
+            "iso_a": float(gi[0]),
+            "iso_b": float(gi[1]),
+            "field": self.grad_field.clone() if self.grad_field is not None else None,
+            "series": self.grad_series.clone() if self.grad_series is not None else None,
+        }
+
+
+def _autograd_function():
+    torch = _torch()
+
+    class CoupledBreakthrough(torch.autograd.Function):
+        """C_out(s), s = 1..nsteps, as a function of (iso_a, iso_b, field, series).
+
+        iso_a / iso_b: 0-d tensors; field: per-cell tensor or None; series: per-step tensor
+        (index = step) or None.  Gradients flow to every tensor input that requires them.
+        """
+
+        @staticmethod
+        def forward(ctx, run, iso_a, iso_b, field=None, series=None):
+            run.set_params(iso_a=float(iso_a), iso_b=float(iso_b),
+                           field=None if field is None else field.detach(),
+                           series=None if series is None else series.detach())
+            ctx.run = run
+            ctx.meta = (iso_a.dtype, iso_a.device, None if field is None else (field.dtype, field.device),
+                        None if series is None else (series.dtype, series.device))
+            return run.forward()
+
+        @staticmethod
+        def backward(ctx, grad_c):
+            run = ctx.run
+            g = run.backward(grad_c.detach().to(torch.float64))
+            adt, adev, fmeta, smeta = ctx.meta
+            ga = torch.tensor(g["iso_a"], dtype=adt, device=adev)
+            gb = torch.tensor(g["iso_b"], dtype=adt, device=adev)
+            gf = None if fmeta is None else g["field"].to(fmeta[1], fmeta[0])
+            gs = None if smeta is None else g["series"].to(smeta[1], smeta[0])
+            return None, ga, gb, gf, gs
+
+    return CoupledBreakthrough
+
+
+class _Lazy:
+    _fn = None
+
+    def __getattr__(self, name):
+        if _Lazy._fn is None:
+            _Lazy._fn = _autograd_function()
+        return getattr(_Lazy._fn, name)
+
+
+# torch.autograd.Function subclass, created on first use so that importing this module needs no torch
+CoupledBreakthrough = _Lazy()
+
+
+def breakthrough_loss_weights(c_out, target) -> np.ndarray:
+    """dL/dC_out for L = sum_s (C_out(s) - C*(s))^2."""
+    return 2.0 * (np.asarray(c_out, np.float64) - np.asarray(target, np.float64))

@@ -1,0 +1,650 @@
+// adjoint.hip -- discrete adjoint of the fused coupled step (k_coupled in coupled.hip).
+//
+// The reference has no adjoint; the map differentiated here is exactly the fused forward step
+// (same pulls, boundary handling, clamps and branches), so the gradients are those of the
+// reference's update sequence (main.cpp:319-336).
+//
+// Representation.  The adjoint state after the reverse of step s is held READER-indexed:
+//     lam_f[q][c] = dL / d(the value cell c pulled along q at the start of step s)
+//     lam_g[q][c] = the same for the D3Q7 pulls,
+// plus lam_q[c] = dL/dq(c) and lam_phi[c] = dL/d(lagged concentration at c).  Every stored
+// population slot of the forward state is read by at most two pulls: its streaming neighbour,
+// or the owner itself through bounce-back, plus the owner's keep-local pull where the opposite
+// source lies outside the box.  The reverse step of cell c first GATHERS the adjoint of the
+// slots it wrote in step s from those readers (its own link word says which reader applies),
+// then back-propagates through its local update and writes the adjoint of what it pulled.
+// No atomics are needed for the populations and every adjoint slot is written once per step.
+//
+// Boundary rules (mirroring how k_coupled folds the boundary conditions into the step):
+//   * x = 0 with an equilibrium NS inlet, or an ADDirichlet AD inlet: the stored values are
+//     constants, so the owner's adjoint of them is zero.
+//   * x = nx-1 with a pressure outlet / Neumann outlet: the outlet cell stores nothing; the
+//     thread of (nx-2, y, z), which wrote the outlet slots, gathers their adjoint and
+//     back-propagates it through the pressure equilibrium / the copied g.
+//   * per-step inlet series: pulls whose source lies on x = 0 read the series value; their
+//     adjoint accumulates into grad_series[step].
+//   * clamps (phi >= min_conc, field in [0, 1], tau >= 0.5001) differentiate the active branch;
+//     on the closed interval [0, 1] the field passes its gradient through.
+#include "coupled_common.hpp"
+
+namespace fmx {
+
+template <int P> struct AdjointPtrs {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    // forward state at the start of the step
+    const S *f;
+    const S *g;
+    const C *q;
+    const C *phi;
+    const std::uint32_t *links;
+    const C *field;
+    const C *tau_field;
+    const double *inlet_series;
+    // adjoint: reader-indexed adjoint of the next step's pulls (in) and of this step's pulls (out)
+    const C *lf_in;
+    const C *lg_in;
+    C *lf_out;
+    C *lg_out;
+    C *lam_q;    // in place: dL/dq after the step -> before the step
+    C *lam_phi;  // in place: dL/d(concentration written by the step) -> dL/d(lagged concentration)
+    double *grad_field;
+    double *grad_series;
+    double *iso_acc;
+    const double *outlet_weight;
+};
+
+// Block sum of three values; lane 0 of the block receives the totals.
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
+    __shared__ double red[3][4];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    const int tid = int(threadIdx.x + threadIdx.y * blockDim.x);
+    const int wave = tid >> 6, lane = tid & 63;
+    const int nwaves = int(blockDim.x * blockDim.y) >> 6;
+    if (lane == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+        red[2][wave] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < nwaves; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+            c += red[2][w];
+        }
+    }
+}
+
+template <int P, int OP>
+__global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
+                                                 i64 widx, double inv_count) {
+    using C = typename Storage<P>::C;
+    int bx, by, bz;
+    block_coords(prm.xcd_remap, prm.gx, prm.gy, prm.gz, bx, by, bz);
+    const int x = bx * prm.block_x + int(threadIdx.x);
+    const int y = by * prm.block_y + int(threadIdx.y);
+    const int z = bz;
+    const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
+    const i64 n = prm.n;
+    double acc_a = 0.0, acc_b = 0.0, acc_s = 0.0;
+
+    if (x < nx && y < ny) {
+        const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
+        const std::uint32_t lk = p.links[cell];
+        const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
+
+        int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
+        bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
+        if (prm.py) {
+            if (!ym_ok) ym += ny;
+            if (!yp_ok) yp -= ny;
+            ym_ok = yp_ok = true;
+        }
+        if (prm.pz) {
+            if (!zm_ok) zm += nz;
+            if (!zp_ok) zp -= nz;
+            zm_ok = zp_ok = true;
+        }
+        const unsigned ys[3] = {unsigned(ym), unsigned(y), unsigned(yp)};
+        const unsigned zs[3] = {unsigned(zm), unsigned(z), unsigned(zp)};
+        const bool yok[3] = {ym_ok, true, yp_ok};
+        const bool zok[3] = {zm_ok, true, zp_ok};
+        // cell (xx, y + dy, z + dz) in the box (after periodic wrap), and its index
+        auto inside = [&](int xx, int dy, int dz) -> bool { return xx >= 0 && xx < nx && yok[1 + dy] && zok[1 + dz]; };
+        auto at = [&](int xx, int dy, int dz) -> unsigned {
+            return unsigned(xx) + unsigned(nx) * (ys[1 + dy] + unsigned(ny) * zs[1 + dz]);
+        };
+
+        // ------------------------------------------------ forward recompute: pulls
+        C fo[19];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
+            bool bb = false;
+            if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
+            if (bb) fo[q] = p.f[i64(Q19::opp[q]) * n + cell];
+            else fo[q] = p.f[i64(q) * n + (inside(x - cx, -cy, -cz) ? at(x - cx, -cy, -cz) : cell)];
+        });
+        const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
+        C gi[7];
+        bool subst[7];
+        static_for<0, 7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+            const bool in = inside(x - cx, -cy, -cz);
+            const int sx = in ? x - cx : x;
+            subst[q] = prm.ad_inlet == 2 && sx == 0;
+            gi[q] = subst[q] ? phi_pre * w7<C>(q) : p.g[i64(q) * n + (in ? at(x - cx, -cy, -cz) : cell)];
+        });
+        const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
+        const C phi_lag = sorbs ? p.phi[cell] : C(0);
+        const C q_old = sorbs ? p.q[cell] : C(0);
+        const C fv = p.field ? p.field[cell] : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
+        const C fcl = fv < C(0) ? C(0) : (fv > C(1) ? C(1) : fv);
+        const bool f_pass = fv >= C(0) && fv <= C(1);
+
+        // ------------------------------------------------ forward recompute: NS collision
+        C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            rho += fo[q];
+            jx = axpy1<Q19::cx[q]>(jx, fo[q]);
+            jy = axpy1<Q19::cy[q]>(jy, fo[q]);
+            jz = axpy1<Q19::cz[q]>(jz, fo[q]);
+        });
+        const bool rho_ok = rho > C(0);
+        C ux = C(0), uy = C(0), uz = C(0), invR = C(0);
+        if (rho_ok) {
+            invR = C(1) / rho;
+            ux = jx * invR;
+            uy = jy * invR;
+            uz = jz * invR;
+            if (OP == FMX_NS_POROUS) {
+                ux *= fcl;
+                uy *= fcl;
+                uz *= fcl;
+            }
+        } else {
+            rho = prm.rho0;
+        }
+        const C u2x15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
+        const C ns = (OP == FMX_NS_PBB) ? fcl : C(0);
+        C fc[19];  // collided populations (before the partial bounce-back blend)
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            fc[q] = fo[q] + prm.omega * (feq_bgk<q>(rho, ux, uy, uz, u2x15) - fo[q]);
+        });
+        // post-collision moments of the stored result
+        C r2 = C(0), vx = C(0), vy = C(0), vz = C(0);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C fr = fc[q];
+            if (OP == FMX_NS_PBB) fr = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
+            if (solid) fr = fo[q];
+            r2 += fr;
+            vx = axpy1<Q19::cx[q]>(vx, fr);
+            vy = axpy1<Q19::cy[q]>(vy, fr);
+            vz = axpy1<Q19::cz[q]>(vz, fr);
+        });
+        const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
+        const C ux2 = vx * inv2, uy2 = vy * inv2, uz2 = vz * inv2;
+
+        // ------------------------------------------------ forward recompute: sorption and AD
+        C ce = C(0), bc = C(0);
+        if (sorbs) {
+            const C c = phi_lag < prm.min_conc ? prm.min_conc : phi_lag;
+            ce = c - prm.min_conc;
+            bc = prm.iso_b * ce;
+        }
+        C phis = C(0);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) phis += gi[q];
+        const bool clamped = phis < prm.min_conc;
+        const C phi = clamped ? prm.min_conc : phis;
+        const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
+        const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
+
+        // ------------------------------------------------ gather the adjoint of what the step stored
+        // adjoint of slot (q, owner) of f: its reader's adjoint (see the header comment)
+        auto gather_f = [&](auto qc, int ox, unsigned oc, std::uint32_t lo) -> C {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q], oq = Q19::opp[q];
+            if (OP == FMX_NS_BGK && (lo & FMX_LINK_SOLID)) return p.lf_in[i64(oq) * n + oc];
+            C v = C(0);
+            if (OP == FMX_NS_BGK && q != 0 && ((lo >> oq) & 1u)) v = p.lf_in[i64(oq) * n + oc];
+            else if (inside(ox + cx, cy, cz)) v = p.lf_in[i64(q) * n + at(ox + cx, cy, cz)];
+            if (!inside(ox - cx, -cy, -cz)) v += p.lf_in[i64(q) * n + oc];
+            return v;
+        };
+        auto gather_g = [&](auto qc, int ox, unsigned oc) -> C {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+            if (prm.ad_inlet == 2 && ox == 0) return C(0);  // every reader takes the series value instead
+            C v = C(0);
+            if (inside(ox + cx, cy, cz)) v = p.lg_in[i64(q) * n + at(ox + cx, cy, cz)];
+            if (!inside(ox - cx, -cy, -cz)) v += p.lg_in[i64(q) * n + oc];
+            return v;
+        };
+
+        const bool f_const = (prm.ns_inlet && x == 0) || (prm.ns_outlet && x == nx - 1);
+        const bool g_const = (prm.ad_inlet == 1 && x == 0) || (prm.ad_outlet && x == nx - 1);
+        const bool writes_outlet_f = prm.ns_outlet && x == nx - 2;
+        const bool writes_outlet_g = prm.ad_outlet && x == nx - 2;
+        C lfr[19];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lfr[q] = f_const ? C(0) : gather_f(qc, x, cell, lk);
+        });
+        C lgo[7];
+        static_for<0, 7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lgo[q] = g_const ? C(0) : gather_g(qc, x, cell);
+            if (writes_outlet_g) lgo[q] += gather_g(qc, x + 1, cell + 1u);
+        });
+        C lux = C(0), luy = C(0), luz = C(0);  // adjoint of the post-collision velocity
+        if (writes_outlet_f) {  // the outlet cell's populations feq(rho_out, u2) (PressureDirichlet.cu:81-95)
+            const std::uint32_t lo = p.links[cell + 1u];
+            C sw = C(0);
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C lp = gather_f(qc, x + 1, cell + 1u, lo) * w19<C>(q) * prm.rho_out;
+                const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux2, uy2, uz2);
+                const C t = lp * C(3) * (C(1) + C(3) * eu);
+                lux = axpy1<Q19::cx[q]>(lux, t);
+                luy = axpy1<Q19::cy[q]>(luy, t);
+                luz = axpy1<Q19::cz[q]>(luz, t);
+                sw += lp;
+            });
+            lux -= C(3) * ux2 * sw;
+            luy -= C(3) * uy2 * sw;
+            luz -= C(3) * uz2 * sw;
+        }
+        const C lq_next = p.lam_q[cell];
+        const C seed = (p.outlet_weight && x == nx - 1) ? C(p.outlet_weight[widx] * inv_count) : C(0);
+        C lphi = p.lam_phi[cell] + seed;
+
+        // ------------------------------------------------ AD collision: go = gi + om (geq - gi) + w S
+        C lS = C(0), lax = C(0), lay = C(0), laz = C(0);
+        C lgi[7];
+        static_for<0, 7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const C w = w7<C>(q);
+            const C eu = dot3<Q7::cx[q], Q7::cy[q], Q7::cz[q]>(ax, ay, az);
+            const C lgeq = om * lgo[q];
+            lS += w * lgo[q];
+            lphi += lgeq * w * (C(1) + C(4) * eu);
+            const C t = lgeq * w * phi * C(4);
+            lax = axpy1<Q7::cx[q]>(lax, t);
+            lay = axpy1<Q7::cy[q]>(lay, t);
+            laz = axpy1<Q7::cz[q]>(laz, t);
+            lgi[q] = (C(1) - om) * lgo[q];
+        });
+        if (!clamped) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q) lgi[q] += lphi;
+        }
+        C ser = C(0);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            if (subst[q]) ser += w7<C>(q) * lgi[q];
+            p.lg_out[i64(q) * n + cell] = lgi[q];
+        }
+        acc_s = double(ser);
+
+        // ------------------------------------------------ sorption: q' = q + dq, S = -dq
+        if (sorbs) {
+            const C ldq = lq_next - lS;
+            const C lqeq = prm.k_ldf * ldq;
+            C lce;
+            if (prm.isotherm == FMX_ISO_LANGMUIR) {
+                const C den = C(1) + bc;
+                acc_a = double(lqeq * (bc / den));
+                const C lbc = lqeq * prm.iso_a / (den * den);
+                acc_b = double(lbc * ce);
+                lce = lbc * prm.iso_b;
+            } else {
+                acc_a = double(lqeq * ce);
+                lce = lqeq * prm.iso_a;
+            }
+            p.lam_q[cell] = lq_next - prm.k_ldf * ldq;
+            p.lam_phi[cell] = phi_lag < prm.min_conc ? C(0) : lce;
+        } else {
+            p.lam_q[cell] = C(0);
+            p.lam_phi[cell] = C(0);
+        }
+
+        // ------------------------------------------------ NS: post-collision velocity u2 = v / r2
+        if (prm.coupling) {
+            lux += lax;
+            luy += lay;
+            luz += laz;
+        }
+        if (r2 > C(0)) {
+            const C lu_u = lux * ux2 + luy * uy2 + luz * uz2;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C lc = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(lux, luy, luz);
+                lfr[q] += (lc - lu_u) * inv2;
+            });
+        }
+
+        // ------------------------------------------------ NS collision and pull
+        C lfo[19];
+        if (solid) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) lfo[q] = lfr[q];
+        } else {
+            C lfield = C(0);
+            C lfc[19];
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lfc[q] = lfr[q];
+                lfo[q] = C(0);
+            });
+            if (OP == FMX_NS_PBB) {  // fr = (1 - ns) fc + ns fo[opp]
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    lfield += lfr[q] * (fo[Q19::opp[q]] - fc[q]);
+                    lfo[Q19::opp[q]] += ns * lfr[q];
+                    lfc[q] = (C(1) - ns) * lfr[q];
+                });
+            }
+            // fc = fo + omega (feq(rho, u) - fo)
+            C lrho = C(0), lvx = C(0), lvy = C(0), lvz = C(0), sw = C(0);
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C lfeq = prm.omega * lfc[q];
+                const C w = w19<C>(q);
+                const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
+                const C cu = C(3) * eu;
+                lrho += lfeq * w * (C(1) + cu + C(0.5) * cu * cu - u2x15);
+                const C t = lfeq * w * rho * C(3) * (C(1) + cu);
+                lvx = axpy1<Q19::cx[q]>(lvx, t);
+                lvy = axpy1<Q19::cy[q]>(lvy, t);
+                lvz = axpy1<Q19::cz[q]>(lvz, t);
+                sw += lfeq * w;
+                lfo[q] += (C(1) - prm.omega) * lfc[q];
+            });
+            C ljx = C(0), ljy = C(0), ljz = C(0);
+            if (rho_ok) {  // the rho <= 0 fallback uses constants
+                lvx -= C(3) * rho * ux * sw;
+                lvy -= C(3) * rho * uy * sw;
+                lvz -= C(3) * rho * uz * sw;
+                // u = s j / rho (s = clamped porosity for the porous operator, else 1)
+                const C s = (OP == FMX_NS_POROUS) ? fcl : C(1);
+                ljx = lvx * s * invR;
+                ljy = lvy * s * invR;
+                ljz = lvz * s * invR;
+                lrho -= (lvx * ux + lvy * uy + lvz * uz) * invR;
+                if (OP == FMX_NS_POROUS) lfield = (lvx * jx + lvy * jy + lvz * jz) * invR;
+            } else {
+                lrho = C(0);
+            }
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lfo[q] += lrho + dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ljx, ljy, ljz);
+            });
+            if ((OP == FMX_NS_PBB || OP == FMX_NS_POROUS) && p.grad_field && p.field && f_pass)
+                p.grad_field[cell] += double(lfield);
+        }
+#pragma unroll
+        for (int q = 0; q < 19; ++q) p.lf_out[i64(q) * n + cell] = lfo[q];
+    }
+
+    // ---------------------------------------------------- parameter gradients
+    const bool want_series = prm.ad_inlet == 2 && p.grad_series && bx == 0;
+    if (!prm.isotherm && !want_series) return;  // block-uniform
+    block_sum3(acc_a, acc_b, acc_s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int slot = int(blockIdx.x % FMX_ADJ_SLOTS);
+        if (prm.isotherm) {
+            unsafeAtomicAdd(p.iso_acc + slot * FMX_ADJ_STRIDE, acc_a);
+            unsafeAtomicAdd(p.iso_acc + slot * FMX_ADJ_STRIDE + 1, acc_b);
+        }
+        if (want_series) unsafeAtomicAdd(p.grad_series + step, acc_s);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_iso_final(const double *__restrict__ acc, double *__restrict__ grad) {
+    const int t = int(threadIdx.x);
+    double a = 0.0, b = 0.0;
+    if (t < FMX_ADJ_SLOTS) {
+        a = acc[t * FMX_ADJ_STRIDE];
+        b = acc[t * FMX_ADJ_STRIDE + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if (t == 0) {
+        grad[0] = a;
+        grad[1] = b;
+    }
+}
+
+// ------------------------------------------------------------- host side
+inline i64 align256(i64 b) { return (b + 255) & ~i64(255); }
+
+// One stored state: f (19 planes), g (7 planes), q, phi, each section 256-byte aligned.
+template <int P> struct StateView {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    S *f;
+    S *g;
+    C *q;
+    C *phi;
+};
+
+template <int P> i64 state_bytes(i64 n) {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    return align256(19 * n * i64(sizeof(S))) + align256(7 * n * i64(sizeof(S))) + 2 * align256(n * i64(sizeof(C)));
+}
+
+template <int P> StateView<P> state_at(void *base, i64 slot, i64 n) {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    char *b = static_cast<char *>(base) + slot * state_bytes<P>(n);
+    StateView<P> v;
+    v.f = reinterpret_cast<S *>(b);
+    b += align256(19 * n * i64(sizeof(S)));
+    v.g = reinterpret_cast<S *>(b);
+    b += align256(7 * n * i64(sizeof(S)));
+    v.q = reinterpret_cast<C *>(b);
+    b += align256(n * i64(sizeof(C)));
+    v.phi = reinterpret_cast<C *>(b);
+    return v;
+}
+
+template <int P> static int save_state(const fmx_coupled_bufs &b, int cur, const StateView<P> &dst, i64 n, hipStream_t s) {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    if (int rc = check_hip(hipMemcpyAsync(dst.f, b.f[cur], 19 * n * sizeof(S), hipMemcpyDeviceToDevice, s), "save f"))
+        return rc;
+    if (int rc = check_hip(hipMemcpyAsync(dst.g, b.g[cur], 7 * n * sizeof(S), hipMemcpyDeviceToDevice, s), "save g"))
+        return rc;
+    if (int rc = check_hip(hipMemcpyAsync(dst.q, b.q, n * sizeof(C), hipMemcpyDeviceToDevice, s), "save q")) return rc;
+    return check_hip(hipMemcpyAsync(dst.phi, b.phi[cur], n * sizeof(C), hipMemcpyDeviceToDevice, s), "save phi");
+}
+
+// One forward step from stored state a into stored state b (no outlet sums, no inlet_raw).
+template <int P>
+static int step_between(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, const StateView<P> &a,
+                        const StateView<P> &o, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                        hipStream_t s) {
+    using C = typename Storage<P>::C;
+    CoupledPtrs<P> ptr{};
+    ptr.f = a.f;
+    ptr.fn = o.f;
+    ptr.g = a.g;
+    ptr.gn = o.g;
+    ptr.q = a.q;
+    ptr.q_new = o.q;
+    ptr.phi = a.phi;
+    ptr.phi_new = o.phi;
+    ptr.links = b.links;
+    ptr.field = static_cast<const C *>(b.field);
+    ptr.tau_field = static_cast<const C *>(b.tau_field);
+    ptr.inlet_raw = nullptr;
+    ptr.inlet_series = b.inlet_series;
+    ptr.outlet_acc = nullptr;
+    return launch_coupled_step<P>(d.ns_op, ptr, prm, step, 0, s);
+}
+
+template <int P>
+static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, const StateView<P> &st,
+                        const fmx_adjoint_bufs &adj, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                        i64 widx, int lam_cur, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    AdjointPtrs<P> ap{};
+    ap.f = st.f;
+    ap.g = st.g;
+    ap.q = st.q;
+    ap.phi = st.phi;
+    ap.links = b.links;
+    ap.field = static_cast<const C *>(b.field);
+    ap.tau_field = static_cast<const C *>(b.tau_field);
+    ap.inlet_series = b.inlet_series;
+    ap.lf_in = static_cast<const C *>(adj.lam_f[lam_cur]);
+    ap.lg_in = static_cast<const C *>(adj.lam_g[lam_cur]);
+    ap.lf_out = static_cast<C *>(adj.lam_f[1 - lam_cur]);
+    ap.lg_out = static_cast<C *>(adj.lam_g[1 - lam_cur]);
+    ap.lam_q = static_cast<C *>(adj.lam_q);
+    ap.lam_phi = static_cast<C *>(adj.lam_phi);
+    ap.grad_field = adj.grad_field;
+    ap.grad_series = adj.grad_series;
+    ap.iso_acc = adj.iso_acc;
+    ap.outlet_weight = adj.outlet_weight;
+    const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
+    const double inv_count = 1.0 / double(i64(d.ny) * d.nz);
+    switch (d.ns_op) {
+        case FMX_NS_BGK:
+            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_BGK>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            break;
+        case FMX_NS_POROUS:
+            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_POROUS>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            break;
+        default:
+            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_PBB>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            break;
+    }
+    return after_launch("k_adjoint");
+}
+
+static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
+                         int32_t k, const char *who) {
+    if (int rc = fmx_coupled_validate(d, b)) return rc;
+    if (d->prec != FMX_F32 && d->prec != FMX_F64)
+        return set_error(FMX_ERR_UNSUPPORTED, "%s: the adjoint runs on FP32 or FP64 state", who);
+    if (d->ns_inlet == 1)
+        return set_error(FMX_ERR_UNSUPPORTED, "%s: the NEQ velocity inlet has no adjoint (use the equilibrium inlet)",
+                         who);
+    if (step0 < 0 || nsteps < 1 || k < 1) return set_error(FMX_ERR_ARG, "%s: bad step range or interval", who);
+    return FMX_OK;
+}
+
+template <int P>
+static int forward_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int k, int cur,
+                        void *ckpt, hipStream_t s) {
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    if (int rc = clear_frozen(d, b.links, s)) return rc;
+    const int nseg = (nsteps + k - 1) / k;
+    for (int j = 0; j < nseg; ++j) {
+        const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
+        if (int rc = save_state<P>(b, cur, state_at<P>(ckpt, j, n), n, s)) return rc;
+        if (int rc = run_coupled_steps<P>(d, b, step0 + i64(j) * k, len, cur, s, i64(j) * k)) return rc;
+        cur = (cur + len) & 1;
+    }
+    return FMX_OK;
+}
+
+template <int P>
+static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int k,
+                    void *ckpt, void *seg, const fmx_adjoint_bufs &adj, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    const CoupledParams<C> prm = make_params<C>(d);
+    if (int rc = clear_frozen(d, b.links, s)) return rc;
+    // terminal condition: nothing after the last step depends on its state
+    struct Z {
+        void *p;
+        size_t bytes;
+    } zeros[] = {{adj.lam_f[0], size_t(19 * n) * sizeof(C)},
+                 {adj.lam_g[0], size_t(7 * n) * sizeof(C)},
+                 {adj.lam_q, size_t(n) * sizeof(C)},
+                 {adj.lam_phi, size_t(n) * sizeof(C)},
+                 {adj.iso_acc, size_t(FMX_ADJ_SLOTS * FMX_ADJ_STRIDE) * sizeof(double)},
+                 {adj.grad_field, adj.grad_field ? size_t(n) * sizeof(double) : 0},
+                 {adj.grad_series ? adj.grad_series + step0 : nullptr, size_t(nsteps) * sizeof(double)}};
+    for (const Z &zz : zeros)
+        if (zz.p && zz.bytes)
+            if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
+    const int nseg = (nsteps + k - 1) / k;
+    int lam = 0;
+    for (int j = nseg - 1; j >= 0; --j) {
+        const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
+        const i64 s0 = step0 + i64(j) * k;
+        // states at the start of steps s0 .. s0+len-1: the checkpoint, then seg slots 0 .. len-2
+        auto state = [&](int i) { return i == 0 ? state_at<P>(ckpt, j, n) : state_at<P>(seg, i - 1, n); };
+        for (int i = 1; i < len; ++i)
+            if (int rc = step_between<P>(d, b, state(i - 1), state(i), prm, s0 + i - 1, s)) return rc;
+        for (int i = len - 1; i >= 0; --i) {
+            if (int rc = adjoint_step<P>(d, b, state(i), adj, prm, s0 + i, s0 + i - step0, lam, s)) return rc;
+            lam ^= 1;
+        }
+    }
+    hipLaunchKernelGGL(k_iso_final, dim3(1), dim3(64), 0, s, adj.iso_acc, adj.grad_iso);
+    return after_launch("k_iso_final");
+}
+
+}  // namespace fmx
+
+using namespace fmx;
+
+extern "C" {
+
+int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *d) {
+    if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1) return -1;
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    switch (d->prec) {
+        case FMX_F32: return state_bytes<FMX_F32>(n);
+        case FMX_F64: return state_bytes<FMX_F64>(n);
+        default: return -1;
+    }
+}
+
+int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
+                        int32_t k, int cur, void *ckpt, void *stream) {
+    FMX_TRY_BEGIN
+    if (int rc = adjoint_check(d, b, step0, nsteps, k, "fmx_adjoint_forward")) return rc;
+    if ((cur != 0 && cur != 1) || !ckpt) return set_error(FMX_ERR_ARG, "fmx_adjoint_forward: bad cur or checkpoint store");
+    hipStream_t s = as_stream(stream);
+    if (d->prec == FMX_F32) return forward_ckpt<FMX_F32>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+    return forward_ckpt<FMX_F64>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+    FMX_TRY_END
+}
+
+int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
+                         int32_t k, const void *ckpt, void *seg, const fmx_adjoint_bufs *adj, void *stream) {
+    FMX_TRY_BEGIN
+    if (int rc = adjoint_check(d, b, step0, nsteps, k, "fmx_adjoint_backward")) return rc;
+    if (!ckpt || (k > 1 && nsteps > 1 && !seg) || !adj)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing checkpoint or segment store");
+    if (!adj->lam_f[0] || !adj->lam_f[1] || !adj->lam_g[0] || !adj->lam_g[1] || !adj->lam_q || !adj->lam_phi ||
+        !adj->iso_acc || !adj->grad_iso)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing adjoint buffer");
+    if (d->ad_inlet != 2 && adj->grad_series)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: grad_series needs the per-step inlet series (ad_inlet 2)");
+    hipStream_t s = as_stream(stream);
+    void *ck = const_cast<void *>(ckpt);
+    if (d->prec == FMX_F32) return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+    return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+    FMX_TRY_END
+}
+
+}  // extern "C"

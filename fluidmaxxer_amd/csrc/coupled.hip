@@ -36,91 +36,9 @@
 //     in place as state (read, then overwritten with this step's value).
 //   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
 //     into FMX_OUTLET_SLOTS FP64 accumulators per step.
-#include "fmx_exact.hpp"
-#include "fmx_internal.hpp"
+#include "coupled_common.hpp"
 
 namespace fmx {
-
-template <class C> struct CoupledParams {
-    int nx, ny, nz, py, pz;
-    int ns_inlet, ns_outlet, ad_inlet, ad_outlet, isotherm, coupling;
-    int gx, gy, gz;          // logical block grid
-    int block_x, block_y;
-    int xcd_remap;
-    i64 n;
-    C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
-};
-
-template <int P> struct CoupledPtrs {
-    using S = typename Storage<P>::S;
-    using C = typename Storage<P>::C;
-    const S *f;
-    S *fn;
-    const S *g;
-    S *gn;
-    C *q;
-    const C *phi;   // previous step's concentration (sorption input, half-storage reference of g)
-    C *phi_new;     // this step's concentration
-    std::uint32_t *links;
-    const C *field;
-    const C *tau_field;
-    C *inlet_raw;
-    const double *inlet_series;
-    double *outlet_acc;
-};
-
-// Logical block id -> (bx, by, bz).  With xcd_remap the hardware's round-robin
-// assignment of consecutive workgroups to the 8 XCDs is inverted, so each XCD
-// sweeps one contiguous slab of the domain and its L2 sees the neighbouring rows
-// its pulls touch.
-__device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, int &bx, int &by, int &bz) {
-    const unsigned total = unsigned(gx) * unsigned(gy) * unsigned(gz);
-    unsigned b = blockIdx.x;
-    if (remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
-    bx = int(b % unsigned(gx));
-    const unsigned r = b / unsigned(gx);
-    by = int(r % unsigned(gy));
-    bz = int(r / unsigned(gy));
-}
-
-// Population plane q of a structure-of-arrays buffer (wave-uniform pointer) and a load /
-// store at a 32-bit byte offset inside it, so the address is a scalar base plus a vector
-// offset (the host guarantees every plane is below 4 GiB).
-template <class T> __device__ __forceinline__ T *plane(T *base, int q, i64 n) { return base + i64(q) * n; }
-template <class T> __device__ __forceinline__ T ld_b(const T *base, unsigned boff) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
-}
-template <class T> __device__ __forceinline__ T ld_b_nt(const T *base, unsigned boff) {
-    return __builtin_nontemporal_load(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
-}
-template <class T> __device__ __forceinline__ void st_b(T *base, unsigned boff, T v) {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
-}
-template <class T> __device__ __forceinline__ void st_b_nt(T *base, unsigned boff, T v) {
-    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
-}
-
-template <int P>
-__device__ __forceinline__ typename Storage<P>::C decode(typename Storage<P>::S v, typename Storage<P>::C w,
-                                                          typename Storage<P>::C ref) {
-    using C = typename Storage<P>::C;
-    if constexpr (IsHalf<P>::value) return w * (ref + C(v));
-    else return v;
-}
-template <int P>
-__device__ __forceinline__ typename Storage<P>::S encode(typename Storage<P>::C v, typename Storage<P>::C inv_w,
-                                                          typename Storage<P>::C ref) {
-    using S = typename Storage<P>::S;
-    if constexpr (IsHalf<P>::value) return S(v * inv_w - ref);
-    else return v;
-}
-
-// feq in the BGK form w rho (1 + cu + 0.5 cu^2 - 1.5 u^2), cu = 3 e.u (BGKCollision.cu:166-173)
-template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy, C uz, C u2x15) {
-    const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
-    const C cu = C(3) * eu;
-    return w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - u2x15);
-}
 
 template <int P, int OP, bool NTL>
 __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
@@ -302,8 +220,10 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     const bool outlet_cell = prm.ns_outlet && x == nx - 1;
     if (inlet_cell) {
         const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
+        if (p.inlet_raw) {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+            for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+        }
         if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
             const C b15 = C(1.5) * prm.uin2;
             static_for<0, 19>([&](auto qc) {
@@ -353,7 +273,7 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
             qeq = prm.iso_a * ce;
         }
         const C dq = prm.k_ldf * (qeq - q_old);
-        p.q[cell] = q_old + dq;
+        p.q_new[cell] = q_old + dq;
         src_term = -dq;
     }
 
@@ -589,63 +509,6 @@ __global__ void __launch_bounds__(256) k_outlet_mean(const double *__restrict__ 
 }
 
 // ------------------------------------------------------------- host side
-struct LaunchShape {
-    int bx = 256, by = 1, remap = 1;
-};
-
-static LaunchShape launch_shape() {
-    LaunchShape s;
-    if (const char *e = std::getenv("FMX_BLOCK")) {
-        int a = 0, b = 0;
-        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 256 && (a * b) % 64 == 0) {
-            s.bx = a;
-            s.by = b;
-        }
-    }
-    if (const char *e = std::getenv("FMX_XCD_REMAP")) s.remap = std::atoi(e) != 0;
-    return s;
-}
-
-template <class C> static CoupledParams<C> make_params(const fmx_coupled_desc &d) {
-    CoupledParams<C> p{};
-    const LaunchShape ls = launch_shape();
-    p.nx = d.nx;
-    p.ny = d.ny;
-    p.nz = d.nz;
-    p.py = d.periodic_y != 0;
-    p.pz = d.periodic_z != 0;
-    p.ns_inlet = d.ns_inlet;
-    p.ns_outlet = d.ns_outlet;
-    p.ad_inlet = d.ad_inlet;
-    p.ad_outlet = d.ad_outlet;
-    p.isotherm = d.isotherm;
-    p.coupling = d.coupling;
-    p.block_x = ls.bx;
-    p.block_y = ls.by;
-    p.gx = (d.nx + ls.bx - 1) / ls.bx;
-    p.gy = (d.ny + ls.by - 1) / ls.by;
-    p.gz = d.nz;
-    p.xcd_remap = ls.remap;
-    p.n = i64(d.nx) * d.ny * d.nz;
-    C tau = C(d.tau);
-    if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204
-    p.omega = C(1) / tau;
-    p.rho0 = C(d.rho0);
-    p.uin_x = C(d.u_in[0]);
-    p.uin_y = C(d.u_in[1]);
-    p.uin_z = C(d.u_in[2]);
-    p.uin2 = p.uin_x * p.uin_x + p.uin_y * p.uin_y + p.uin_z * p.uin_z;
-    p.rho_out = C(d.rho_out);
-    p.phi_in = C(d.phi_in);
-    p.omega_ad = C(1) / C(d.tau_ad);
-    p.min_conc = C(d.min_conc);
-    p.iso_a = C(d.iso_a);
-    p.iso_b = C(d.iso_b);
-    p.k_ldf = C(d.k_ldf);
-    p.f_ref = C(d.f_ref);
-    return p;
-}
-
 template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int cur) {
     using S = typename Storage<P>::S;
     using C = typename Storage<P>::C;
@@ -654,7 +517,8 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     p.fn = static_cast<S *>(b.f[1 - cur]);
     p.g = static_cast<const S *>(b.g[cur]);
     p.gn = static_cast<S *>(b.g[1 - cur]);
-    p.q = static_cast<C *>(b.q);
+    p.q = static_cast<const C *>(b.q);
+    p.q_new = static_cast<C *>(b.q);
     p.phi = static_cast<const C *>(b.phi[cur]);
     p.phi_new = static_cast<C *>(b.phi[1 - cur]);
     p.links = b.links;
@@ -666,38 +530,58 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     return p;
 }
 
-template <int P, int OP>
-static int run_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
-                     hipStream_t s) {
+template <int P>
+int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
+                        i64 step, i64 acc_row, hipStream_t s) {
+    const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
+    // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads (measured slower; kept for study)
+    static const bool nt_loads = [] {
+        const char *e = std::getenv("FMX_NT_LOADS");
+        return e != nullptr && std::atoi(e) != 0;
+    }();
+#define FMX_LAUNCH(OP, NT) hipLaunchKernelGGL((k_coupled<P, OP, NT>), grid, block, 0, s, ptr, prm, step, acc_row)
+    switch (ns_op) {
+        case FMX_NS_BGK:
+            if (nt_loads) FMX_LAUNCH(FMX_NS_BGK, true);
+            else FMX_LAUNCH(FMX_NS_BGK, false);
+            break;
+        case FMX_NS_POROUS:
+            if (nt_loads) FMX_LAUNCH(FMX_NS_POROUS, true);
+            else FMX_LAUNCH(FMX_NS_POROUS, false);
+            break;
+        case FMX_NS_PBB:
+            if (nt_loads) FMX_LAUNCH(FMX_NS_PBB, true);
+            else FMX_LAUNCH(FMX_NS_PBB, false);
+            break;
+        default: return set_error(FMX_ERR_ARG, "fmx_coupled: unknown ns_op %d", ns_op);
+    }
+#undef FMX_LAUNCH
+    return after_launch("k_coupled");
+}
+
+template int launch_coupled_step<FMX_F32>(int, const CoupledPtrs<FMX_F32> &, const CoupledParams<float> &, i64, i64,
+                                          hipStream_t);
+template int launch_coupled_step<FMX_F64>(int, const CoupledPtrs<FMX_F64> &, const CoupledParams<double> &, i64, i64,
+                                          hipStream_t);
+
+// Steps step0 .. step0+nsteps-1 on the ping-pong buffers; the outlet sums of step step0+k go to row
+// acc_row0 + k of outlet_acc.
+template <int P>
+int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
+                      hipStream_t s, i64 acc_row0) {
     using C = typename Storage<P>::C;
     const CoupledParams<C> prm = make_params<C>(d);
-    const unsigned nblocks = unsigned(prm.gx) * unsigned(prm.gy) * unsigned(prm.gz);
-    const dim3 block(prm.block_x, prm.block_y, 1);
-    // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads
-    const char *ntl = std::getenv("FMX_NT_LOADS");
-    const bool nt_loads = ntl != nullptr && std::atoi(ntl) != 0;
     for (int k = 0; k < nsteps; ++k) {
         const CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
-        if (nt_loads)
-            hipLaunchKernelGGL((k_coupled<P, OP, true>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
-        else
-            hipLaunchKernelGGL((k_coupled<P, OP, false>), dim3(nblocks), block, 0, s, ptr, prm, step0 + k, i64(k));
-        const int rc = after_launch("k_coupled");
-        if (rc) return rc;
+        if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, acc_row0 + k, s)) return rc;
     }
     return FMX_OK;
 }
 
-template <int P>
-static int run_prec(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
-                    hipStream_t s) {
-    switch (d.ns_op) {
-        case FMX_NS_BGK: return run_steps<P, FMX_NS_BGK>(d, b, step0, nsteps, cur, s);
-        case FMX_NS_POROUS: return run_steps<P, FMX_NS_POROUS>(d, b, step0, nsteps, cur, s);
-        case FMX_NS_PBB: return run_steps<P, FMX_NS_PBB>(d, b, step0, nsteps, cur, s);
-        default: return set_error(FMX_ERR_ARG, "fmx_coupled_run: unknown ns_op %d", d.ns_op);
-    }
-}
+template int run_coupled_steps<FMX_F32>(const fmx_coupled_desc &, const fmx_coupled_bufs &, i64, int, int, hipStream_t,
+                                        i64);
+template int run_coupled_steps<FMX_F64>(const fmx_coupled_desc &, const fmx_coupled_bufs &, i64, int, int, hipStream_t,
+                                        i64);
 
 template <int P>
 static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s) {
@@ -718,6 +602,12 @@ static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
     const int allow = fz != nullptr && std::atoi(fz) != 0;
     hipLaunchKernelGGL(k_freeze<P>, dim3(blocks_for(prm.n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
                        static_cast<S *>(b.f[1 - cur]), b.links, prm, allow);
+    return after_launch("k_freeze");
+}
+
+int clear_frozen(const fmx_coupled_desc &d, std::uint32_t *links, hipStream_t s) {
+    const CoupledParams<float> prm = make_params<float>(d);
+    hipLaunchKernelGGL(k_freeze<FMX_F32>, dim3(blocks_for(prm.n)), dim3(256), 0, s, nullptr, nullptr, links, prm, 0);
     return after_launch("k_freeze");
 }
 
@@ -864,10 +754,10 @@ int fmx_coupled_run(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_
     if (nsteps < 0 || step0 < 0) return set_error(FMX_ERR_ARG, "fmx_coupled_run: bad step range");
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
-        case FMX_F32: return run_prec<FMX_F32>(*d, *b, step0, nsteps, cur, s);
-        case FMX_F64: return run_prec<FMX_F64>(*d, *b, step0, nsteps, cur, s);
-        case FMX_F16: return run_prec<FMX_F16>(*d, *b, step0, nsteps, cur, s);
-        default: return run_prec<FMX_BF16>(*d, *b, step0, nsteps, cur, s);
+        case FMX_F32: return run_coupled_steps<FMX_F32>(*d, *b, step0, nsteps, cur, s);
+        case FMX_F64: return run_coupled_steps<FMX_F64>(*d, *b, step0, nsteps, cur, s);
+        case FMX_F16: return run_coupled_steps<FMX_F16>(*d, *b, step0, nsteps, cur, s);
+        default: return run_coupled_steps<FMX_BF16>(*d, *b, step0, nsteps, cur, s);
     }
     FMX_TRY_END
 }

@@ -1,0 +1,171 @@
+// coupled_common.hpp -- parameters, pointers, addressing and launch shape shared by the fused
+// coupled step (coupled.hip) and its discrete adjoint (adjoint.hip).
+#pragma once
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "fmx_exact.hpp"
+#include "fmx_internal.hpp"
+
+namespace fmx {
+
+template <class C> struct CoupledParams {
+    int nx, ny, nz, py, pz;
+    int ns_inlet, ns_outlet, ad_inlet, ad_outlet, isotherm, coupling;
+    int gx, gy, gz;          // logical block grid
+    int block_x, block_y;
+    int xcd_remap;
+    i64 n;
+    C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
+};
+
+template <int P> struct CoupledPtrs {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    const S *f;
+    S *fn;
+    const S *g;
+    S *gn;
+    const C *q;     // adsorbed load at the start of the step
+    C *q_new;       // adsorbed load after the step (may alias q: each cell touches only its own)
+    const C *phi;   // previous step's concentration (sorption input, half-storage reference of g)
+    C *phi_new;     // this step's concentration
+    std::uint32_t *links;
+    const C *field;
+    const C *tau_field;
+    C *inlet_raw;   // nullable: x = 0 populations as the reference holds them between steps
+    const double *inlet_series;
+    double *outlet_acc;
+};
+
+// Logical block id -> (bx, by, bz).  With xcd_remap the hardware's round-robin
+// assignment of consecutive workgroups to the 8 XCDs is inverted, so each XCD
+// sweeps one contiguous slab of the domain and its L2 sees the neighbouring rows
+// its pulls touch.
+__device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, int &bx, int &by, int &bz) {
+    const unsigned total = unsigned(gx) * unsigned(gy) * unsigned(gz);
+    unsigned b = blockIdx.x;
+    if (remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
+    bx = int(b % unsigned(gx));
+    const unsigned r = b / unsigned(gx);
+    by = int(r % unsigned(gy));
+    bz = int(r / unsigned(gy));
+}
+
+// Population plane q of a structure-of-arrays buffer (wave-uniform pointer) and a load /
+// store at a 32-bit byte offset inside it, so the address is a scalar base plus a vector
+// offset (the host guarantees every plane is below 4 GiB).
+template <class T> __device__ __forceinline__ T *plane(T *base, int q, i64 n) { return base + i64(q) * n; }
+template <class T> __device__ __forceinline__ T ld_b(const T *base, unsigned boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T> __device__ __forceinline__ T ld_b_nt(const T *base, unsigned boff) {
+    return __builtin_nontemporal_load(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
+}
+template <class T> __device__ __forceinline__ void st_b(T *base, unsigned boff, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+template <class T> __device__ __forceinline__ void st_b_nt(T *base, unsigned boff, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
+}
+
+template <int P>
+__device__ __forceinline__ typename Storage<P>::C decode(typename Storage<P>::S v, typename Storage<P>::C w,
+                                                          typename Storage<P>::C ref) {
+    using C = typename Storage<P>::C;
+    if constexpr (IsHalf<P>::value) return w * (ref + C(v));
+    else return v;
+}
+template <int P>
+__device__ __forceinline__ typename Storage<P>::S encode(typename Storage<P>::C v, typename Storage<P>::C inv_w,
+                                                          typename Storage<P>::C ref) {
+    using S = typename Storage<P>::S;
+    if constexpr (IsHalf<P>::value) return S(v * inv_w - ref);
+    else return v;
+}
+
+// feq in the BGK form w rho (1 + cu + 0.5 cu^2 - 1.5 u^2), cu = 3 e.u (BGKCollision.cu:166-173)
+template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy, C uz, C u2x15) {
+    const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
+    const C cu = C(3) * eu;
+    return w19<C>(q) * rho * (C(1) + cu + C(0.5) * cu * cu - u2x15);
+}
+
+// ------------------------------------------------------------- host side
+// One wave-multiple row of x per block: 256 x 1 for long rows, 128 x 2 / 64 x 4 for short ones,
+// so no lane idles when nx is 64 or 128.  FMX_BLOCK=AxB and FMX_XCD_REMAP=0|1 override.
+struct LaunchShape {
+    int bx = 256, by = 1, remap = 1;
+};
+
+inline LaunchShape launch_shape(int nx) {
+    LaunchShape s;
+    const int rx = ((nx + 63) / 64) * 64;
+    s.bx = rx < 256 ? rx : 256;
+    s.by = 256 / s.bx;
+    if (const char *e = std::getenv("FMX_BLOCK")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 256 && (a * b) % 64 == 0) {
+            s.bx = a;
+            s.by = b;
+        }
+    }
+    if (const char *e = std::getenv("FMX_XCD_REMAP")) s.remap = std::atoi(e) != 0;
+    return s;
+}
+
+template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
+    CoupledParams<C> p{};
+    const LaunchShape ls = launch_shape(d.nx);
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.nz = d.nz;
+    p.py = d.periodic_y != 0;
+    p.pz = d.periodic_z != 0;
+    p.ns_inlet = d.ns_inlet;
+    p.ns_outlet = d.ns_outlet;
+    p.ad_inlet = d.ad_inlet;
+    p.ad_outlet = d.ad_outlet;
+    p.isotherm = d.isotherm;
+    p.coupling = d.coupling;
+    p.block_x = ls.bx;
+    p.block_y = ls.by;
+    p.gx = (d.nx + ls.bx - 1) / ls.bx;
+    p.gy = (d.ny + ls.by - 1) / ls.by;
+    p.gz = d.nz;
+    p.xcd_remap = ls.remap;
+    p.n = i64(d.nx) * d.ny * d.nz;
+    C tau = C(d.tau);
+    if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204
+    p.omega = C(1) / tau;
+    p.rho0 = C(d.rho0);
+    p.uin_x = C(d.u_in[0]);
+    p.uin_y = C(d.u_in[1]);
+    p.uin_z = C(d.u_in[2]);
+    p.uin2 = p.uin_x * p.uin_x + p.uin_y * p.uin_y + p.uin_z * p.uin_z;
+    p.rho_out = C(d.rho_out);
+    p.phi_in = C(d.phi_in);
+    p.omega_ad = C(1) / C(d.tau_ad);
+    p.min_conc = C(d.min_conc);
+    p.iso_a = C(d.iso_a);
+    p.iso_b = C(d.iso_b);
+    p.k_ldf = C(d.k_ldf);
+    p.f_ref = C(d.f_ref);
+    return p;
+}
+
+inline unsigned coupled_blocks(int gx, int gy, int gz) { return unsigned(gx) * unsigned(gy) * unsigned(gz); }
+
+// The fused forward step (coupled.hip): one launch over the whole box.
+template <int P>
+int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
+                        i64 step, i64 acc_row, hipStream_t s);
+// nsteps fused steps on the ping-pong buffers of b, outlet sums of step step0+k into row acc_row0+k.
+template <int P>
+int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
+                      hipStream_t s, i64 acc_row0 = 0);
+// Clears the frozen-solid bits of the link words (every cell back on the full update path).
+int clear_frozen(const fmx_coupled_desc &d, std::uint32_t *links, hipStream_t s);
+
+}  // namespace fmx

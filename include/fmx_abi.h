@@ -220,6 +220,46 @@ int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *buf
 /* OutletCSV::log average (OutletCSV.hpp:51-80) for nsteps steps from outlet_acc: mean[k] = sum / count */
 int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, double *mean, void *stream);
 
+/* ------------------------------------------------------ discrete adjoint */
+/* No counterpart in the reference (it has no adjoint): the exact reverse-mode derivative of the
+ * fused step above (fmx_coupled_run), for FP32 or FP64 state, equilibrium or no NS inlet.
+ *
+ * Loss: any L whose dependence on the run is through the outlet means C_out(s) (fmx_outlet_mean);
+ * the caller supplies outlet_weight[s - step0] = dL/dC_out(s).  Gradients returned:
+ *   grad_iso[0..1]  dL/d iso_a, dL/d iso_b (Langmuir q_max and K_L; linear: K_Henry, -)
+ *   grad_field[c]   dL/d field[c] (partial-bounce-back fraction or porosity; clamps pass the gradient
+ *                   on [0, 1])
+ *   grad_series[s]  dL/d inlet_series[s] (ad_inlet = 2)
+ * Checkpointing: fmx_adjoint_forward keeps the state before steps step0, step0+k, ... in `ckpt`
+ * (ceil(nsteps/k) slots of fmx_adjoint_state_bytes); fmx_adjoint_backward recomputes each segment
+ * into `seg` (k-1 slots) and runs the adjoint steps over it in reverse. */
+#define FMX_ADJ_SLOTS 64
+#define FMX_ADJ_STRIDE 16 /* doubles between accumulator slots */
+
+typedef struct fmx_adjoint_bufs {
+    void *lam_f[2];              /* 19*N each, compute precision: adjoint populations (scratch) */
+    void *lam_g[2];              /* 7*N each (scratch) */
+    void *lam_q, *lam_phi;       /* N each (scratch) */
+    double *iso_acc;             /* FMX_ADJ_SLOTS*FMX_ADJ_STRIDE doubles (scratch) */
+    const double *outlet_weight; /* nsteps: dL/dC_out(step0 + i); NULL = no loss (all gradients 0) */
+    double *grad_iso;            /* 2 doubles (written) */
+    double *grad_field;          /* N doubles (written) or NULL */
+    double *grad_series;         /* indexed like inlet_series; entries step0 .. step0+nsteps-1 written, or NULL */
+} fmx_adjoint_bufs;
+
+/* Bytes of one stored state (f, g, q, phi; each section 256-byte aligned); -1 for half storage */
+int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *desc);
+/* fmx_coupled_run with checkpoints: copies the state before steps step0 + j*k into slot j of ckpt,
+ * outlet sums as fmx_coupled_run (row s - step0); result in buffer (cur + nsteps) % 2.  Clears the
+ * frozen-solid link bits (the adjoint needs every cell on the full update path). */
+int fmx_adjoint_forward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
+                        int32_t k, int cur, void *ckpt, void *stream);
+/* Reverse sweep over the steps of fmx_adjoint_forward (same desc, links, field, tau_field and
+ * inlet_series; f/g/q/phi of bufs are not used).  Segment j is recomputed from ckpt slot j into
+ * seg, then its adjoint steps run last to first. */
+int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
+                         int32_t k, const void *ckpt, void *seg, const fmx_adjoint_bufs *adj, void *stream);
+
 /* ---------------------------------------------------------- diagnostics */
 /* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);
  * f is SoA in `prec` storage (half storage needs f_ref); any input may be NULL (sum = 0) */

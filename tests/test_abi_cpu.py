@@ -46,6 +46,44 @@ def test_validate_rejects_bad_descriptors():
         abi.call("fmx_coupled_run", C.byref(d), C.byref(b), 1, 1, 0, None)
 
 
+def test_adjoint_state_bytes_and_argument_checks():
+    n = 8 * 4 * 4
+    cfg = CoupledConfig(8, 4, 4, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
+    L = abi.lib()
+
+    def al(b):
+        return (b + 255) // 256 * 256
+
+    for prec, s in ((abi.F32, 4), (abi.F64, 8)):
+        d = cfg.desc(prec)
+        assert L.fmx_adjoint_state_bytes(C.byref(d)) == al(19 * n * s) + al(7 * n * s) + 2 * al(n * s)
+    d = cfg.desc(abi.F16)
+    assert L.fmx_adjoint_state_bytes(C.byref(d)) == -1
+    b = abi.CoupledBufs()
+    for i in range(2):
+        b.f[i] = b.g[i] = b.phi[i] = 0x1000
+    b.q = b.links = b.inlet_raw = 0x1000
+    a = abi.AdjointBufs()
+    for i in range(2):
+        a.lam_f[i] = a.lam_g[i] = 0x1000
+    a.lam_q = a.lam_phi = a.iso_acc = a.grad_iso = 0x1000
+
+    def backward(desc, k=5):
+        return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, 0x1000, C.byref(a), None)
+
+    assert backward(d) != 0 and b"FP32 or FP64" in L.fmx_last_error()
+    d = cfg.desc(abi.F32)
+    d.ns_inlet = 1
+    assert backward(d) != 0 and b"NEQ" in L.fmx_last_error()
+    d.ns_inlet = 2
+    assert backward(d, k=0) != 0 and b"interval" in L.fmx_last_error()
+    a.grad_series = 0x1000
+    assert backward(d) != 0 and b"grad_series" in L.fmx_last_error()
+    a.grad_series = None
+    a.grad_iso = None
+    assert backward(d) != 0 and b"adjoint buffer" in L.fmx_last_error()
+
+
 def test_sphere_pack_deterministic_and_on_target():
     a = geometry.sphere_pack(64, 32, 32, 1, 4, 4, 0.6, overlap=False)
     b = geometry.sphere_pack(64, 32, 32, 1, 4, 4, 0.6, overlap=False)

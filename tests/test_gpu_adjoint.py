@@ -1,0 +1,285 @@
+"""Discrete adjoint of the fused coupled step: gradients of an outlet breakthrough-curve
+loss L = sum_s (C_out(s) - C*(s))^2 with respect to the Langmuir constants (q_max, K_L),
+the per-cell partial-bounce-back fraction / porosity and the per-step inlet
+concentration, checked against
+
+  * central finite differences of the GPU forward run in FP64 (relative 1e-6; the
+    FP64 fused step is bit-identical to the FP64 oracle),
+  * central finite differences of the CPU oracle (the restatement of the reference
+    update) in FP64 (relative 1e-6),
+  * the FP64 adjoint, for the FP32 adjoint (relative 1e-3),
+  * itself under a different checkpoint interval (relative 1e-12),
+and used through the torch.autograd.Function wrapper.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fluidmaxxer_amd import workloads
+from fluidmaxxer_amd.adjoint import AdjointRun, CoupledBreakthrough
+from fluidmaxxer_amd.coupled import CoupledConfig
+from fluidmaxxer_amd.geometry import sphere_pack
+from oracle import ref_cpu as R
+from tests.gpu_util import host, say
+from tests.lattice import oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS, K = 400, 64
+
+
+def adj_config(op="pbb", series=True, closed=False, seed=21, dims=(16, 10, 10)):
+    """A small sphere pack whose concentration front reaches the outlet within NSTEPS."""
+    nx, ny, nz = dims
+    pack = sphere_pack(nx, ny, nz, seed, 1.5, 2.5, 0.55, overlap=False, clearance=3)
+    cfg = CoupledConfig(nx, ny, nz, solid=pack.solid, region=pack.sphere, tau=0.7, u_in=(0.08, 0.0, 0.0),
+                        tau_ad=0.8, isotherm="langmuir", iso_a=2.0, iso_b=5.0, k_ldf=0.05, ns_inlet="eq")
+    rng = np.random.default_rng(seed)
+    if op == "pbb":
+        cfg.ns_op = "pbb"
+        cfg.field = 0.05 + 0.9 * pack.solid + rng.uniform(-0.03, 0.03, cfg.n)   # strictly inside (0, 1)
+        cfg.solid = np.zeros_like(pack.solid)
+    elif op == "porous":
+        cfg.ns_op = "porous"
+        cfg.field = np.where(pack.solid == 1, 0.6, 0.95) + rng.uniform(-0.03, 0.03, cfg.n)
+    if series:
+        cfg.ad_inlet = "series"
+        cfg.inlet_series = 0.2 + 0.8 * workloads.series_per_step(workloads.series_knots(9, seed), NSTEPS)
+    if closed:
+        cfg.ns_inlet = None
+        cfg.ns_outlet = False
+        cfg.ad_outlet = False
+        cfg.ad_inlet = "none"
+        cfg.inlet_series = None
+        cfg.phi0 = 0.4
+    return cfg
+
+
+class Problem:
+    """Loss, adjoint gradients and finite differences for one configuration."""
+
+    def __init__(self, cfg, prec="f64", nsteps=NSTEPS, k=K, target=None):
+        say(f"adjoint problem prec={prec} op={cfg.ns_op} dims={(cfg.nx, cfg.ny, cfg.nz)} steps={nsteps} k={k}")
+        self.cfg = cfg
+        self.run = AdjointRun(cfg, nsteps, k, prec)
+        self.base = dict(iso_a=cfg.iso_a, iso_b=cfg.iso_b,
+                         field=None if cfg.field is None else np.asarray(cfg.field, np.float64).copy(),
+                         series=None if cfg.inlet_series is None else np.asarray(cfg.inlet_series, np.float64).copy())
+        c = host(self.run.forward())
+        assert np.all(np.isfinite(c))
+        self.c0 = c
+        self.target = target if target is not None else 0.85 * c + 0.01 * np.sin(np.arange(nsteps) / 17.0)
+
+    def loss(self, **over):
+        p = dict(self.base)
+        p.update(over)
+        self.run.set_params(**{k: v for k, v in p.items() if v is not None})
+        c = host(self.run.forward())
+        return float(np.sum((c - self.target) ** 2))
+
+    def grads(self):
+        self.run.set_params(**{k: v for k, v in self.base.items() if v is not None})
+        c = host(self.run.forward())
+        g = self.run.backward(2.0 * (c - self.target))
+        return {k: (host(v) if hasattr(v, "detach") else v) for k, v in g.items()}
+
+    def fd_scalar(self, name, h):
+        v = self.base[name]
+        return (self.loss(**{name: v + h}) - self.loss(**{name: v - h})) / (2 * h)
+
+    def fd_dir(self, name, direction, h):
+        v = self.base[name]
+        return (self.loss(**{name: v + h * direction}) - self.loss(**{name: v - h * direction})) / (2 * h)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def pbb_problem():
+    return Problem(adj_config("pbb"))
+
+
+def test_front_reaches_outlet(pbb_problem):
+    # the loss must see a real breakthrough curve, not the clamped floor
+    assert pbb_problem.c0[-1] > 0.05, pbb_problem.c0[-1]
+
+
+def test_iso_grads_match_fd_pbb(pbb_problem):
+    g = pbb_problem.grads()
+    for name in ("iso_a", "iso_b"):
+        fd = pbb_problem.fd_scalar(name, 1e-5 * pbb_problem.base[name])
+        say(f"  d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < 1e-6
+
+
+def test_field_grads_match_fd_pbb(pbb_problem):
+    g = pbb_problem.grads()
+    gf = g["field"]
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(pbb_problem.cfg.n)
+    fd = pbb_problem.fd_dir("field", v, 1e-6)
+    ad = float(gf @ v)
+    say(f"  <dL/dfield, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
+    assert rel(ad, fd) < 1e-6
+    for c in np.argsort(-np.abs(gf))[:3]:
+        e = np.zeros(pbb_problem.cfg.n)
+        e[c] = 1.0
+        fdc = pbb_problem.fd_dir("field", e, 1e-6)
+        say(f"  d L/d field[{c}]: adjoint {gf[c]:.12e} fd {fdc:.12e} rel {rel(gf[c], fdc):.2e}")
+        assert rel(gf[c], fdc) < 1e-5
+
+
+def test_series_grads_match_fd(pbb_problem):
+    g = pbb_problem.grads()
+    gs = g["series"]
+    assert gs[0] == 0.0 and np.count_nonzero(gs[1:]) > NSTEPS // 2
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(gs.size)
+    fd = pbb_problem.fd_dir("series", v, 1e-6)
+    ad = float(gs @ v)
+    say(f"  <dL/dseries, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
+    assert rel(ad, fd) < 1e-6
+    for s in np.argsort(-np.abs(gs))[:3]:
+        e = np.zeros(gs.size)
+        e[s] = 1.0
+        fds = pbb_problem.fd_dir("series", e, 1e-5)
+        assert rel(gs[s], fds) < 1e-5, (s, gs[s], fds)
+
+
+@pytest.mark.parametrize("op", ["bgk", "porous"])
+def test_grads_match_fd_other_operators(op):
+    # BGK: bounce-back solids and the ADDirichlet inlet; porous: the porosity gradient
+    pr = Problem(adj_config(op, series=(op == "porous")))
+    g = pr.grads()
+    for name in ("iso_a", "iso_b"):
+        fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
+        say(f"  {op} d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < 1e-6
+    if op == "porous":
+        v = np.random.default_rng(5).standard_normal(pr.cfg.n)
+        fd = pr.fd_dir("field", v, 1e-6)
+        ad = float(g["field"] @ v)
+        say(f"  porous <dL/dfield, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
+        assert rel(ad, fd) < 1e-6
+
+
+def test_grads_match_fd_closed_box():
+    # no inlet/outlet conditions: every x = 0 / x = nx-1 slot has the keep-local second reader
+    pr = Problem(adj_config("pbb", closed=True), target=None)
+    pr.target = pr.c0 * 1.05
+    g = pr.grads()
+    for name in ("iso_a", "iso_b"):
+        fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
+        say(f"  closed d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < 1e-6
+    v = np.random.default_rng(6).standard_normal(pr.cfg.n)
+    fd = pr.fd_dir("field", v, 1e-6)
+    ad = float(g["field"] @ v)
+    assert rel(ad, fd) < 1e-6, (ad, fd)
+
+
+def test_grads_match_oracle_fd(pbb_problem):
+    """Central differences of the CPU oracle (FP64 restatement of the reference update)."""
+    cfg = pbb_problem.cfg
+    outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
+    target = pbb_problem.target
+
+    def oracle_loss(**over):
+        c = dataclasses.replace(cfg, **over)
+        sysm = R.System(oracle_setup(c, R), 8)
+        cr, _, _, _ = sysm.run(NSTEPS, outlet)
+        sysm.close()
+        return float(np.sum((np.asarray(cr) - target) ** 2))
+
+    g = pbb_problem.grads()
+    for name, attr in (("iso_a", "iso_a"), ("iso_b", "iso_b")):
+        v0 = getattr(cfg, attr)
+        h = 1e-5 * v0
+        fd = (oracle_loss(**{attr: v0 + h}) - oracle_loss(**{attr: v0 - h})) / (2 * h)
+        say(f"  oracle fd d L/d {name}: adjoint {g[name]:.12e} oracle {fd:.12e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < 1e-6
+    c = int(np.argmax(np.abs(g["field"])))
+    f0 = np.asarray(cfg.field, np.float64)
+    fp, fm = f0.copy(), f0.copy()
+    fp[c] += 1e-6
+    fm[c] -= 1e-6
+    fd = (oracle_loss(field=fp) - oracle_loss(field=fm)) / 2e-6
+    say(f"  oracle fd d L/d field[{c}]: adjoint {g['field'][c]:.12e} oracle {fd:.12e}")
+    assert rel(g["field"][c], fd) < 1e-5
+
+
+def test_fp32_adjoint_matches_fp64(pbb_problem):
+    g64 = pbb_problem.grads()
+    p32 = Problem(adj_config("pbb"), prec="f32", target=pbb_problem.target)
+    g32 = p32.grads()
+    for name in ("iso_a", "iso_b"):
+        say(f"  f32 vs f64 d L/d {name}: {g32[name]:.8e} vs {g64[name]:.8e} rel {rel(g32[name], g64[name]):.2e}")
+        assert rel(g32[name], g64[name]) < 1e-3
+    for name in ("field", "series"):
+        e = np.linalg.norm(g32[name] - g64[name]) / np.linalg.norm(g64[name])
+        say(f"  f32 vs f64 {name}: relative l2 {e:.2e}")
+        assert e < 1e-3
+
+
+@pytest.mark.parametrize("k", [1, 7, NSTEPS])
+def test_checkpoint_interval_invariance(pbb_problem, k):
+    g = pbb_problem.grads()
+    pr = Problem(adj_config("pbb"), k=k, target=pbb_problem.target)
+    gk = pr.grads()
+    for name in ("iso_a", "iso_b"):
+        assert rel(gk[name], g[name]) < 1e-12, (k, name)
+    for name in ("field", "series"):
+        assert np.max(np.abs(gk[name] - g[name])) <= 1e-12 * np.max(np.abs(g[name])), (k, name)
+
+
+def test_autograd_function(pbb_problem):
+    import torch
+
+    cfg = pbb_problem.cfg
+    run = AdjointRun(cfg, NSTEPS, K, "f64")
+    qmax = torch.tensor(cfg.iso_a, dtype=torch.float64, requires_grad=True)
+    kl = torch.tensor(cfg.iso_b, dtype=torch.float64, requires_grad=True)
+    field = torch.tensor(np.asarray(cfg.field), dtype=torch.float64, device="cuda", requires_grad=True)
+    series = torch.tensor(np.asarray(cfg.inlet_series), dtype=torch.float64, device="cuda", requires_grad=True)
+    target = torch.tensor(pbb_problem.target, device="cuda")
+    c = CoupledBreakthrough.apply(run, qmax, kl, field, series)
+    loss = ((c - target) ** 2).sum()
+    loss.backward()
+    g = pbb_problem.grads()
+    assert rel(qmax.grad.item(), g["iso_a"]) < 1e-12
+    assert rel(kl.grad.item(), g["iso_b"]) < 1e-12
+    assert np.allclose(host(field.grad), g["field"], rtol=1e-12, atol=1e-300)
+    assert np.allclose(host(series.grad), g["series"], rtol=1e-12, atol=1e-300)
+    assert abs(loss.item() - pbb_problem.loss()) <= 1e-12 * loss.item()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_c4_differentiation_workload(prec):
+    """C4: 128x64x64 partial-bounce-back Langmuir pack, 4000 steps, checkpoint every 200; the
+    target curve is the forward run with q_max x 1.1 and K_L x 0.9.  Gradients against FP64
+    central differences of the GPU forward run (FP64 adjoint 1e-6, FP32 adjoint 1e-4)."""
+    steps = 4000
+    cfg = workloads.build("C4", steps=steps)
+    tcfg = dataclasses.replace(cfg, iso_a=cfg.iso_a * 1.1, iso_b=cfg.iso_b * 0.9)
+    trun = AdjointRun(tcfg, steps, 200, "f64")
+    target = host(trun.forward())
+    del trun
+    ref = Problem(cfg, prec="f64", nsteps=steps, k=200, target=target)
+    pr = ref if prec == "f64" else Problem(cfg, prec="f32", nsteps=steps, k=200, target=target)
+    g = pr.grads()
+    say(f"  C4 {prec}: c_out[-1]={pr.c0[-1]:.4e} target[-1]={target[-1]:.4e} "
+        f"dL/dq_max={g['iso_a']:.6e} dL/dK_L={g['iso_b']:.6e}")
+    assert pr.c0[-1] > 1e-3
+    tol = 1e-6 if prec == "f64" else 1e-4
+    for name in ("iso_a", "iso_b"):
+        fd = ref.fd_scalar(name, 1e-5 * ref.base[name])
+        say(f"    d L/d {name}: adjoint {g[name]:.10e} fd(f64) {fd:.10e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < tol
+    v = np.random.default_rng(7).standard_normal(cfg.n)
+    fd = ref.fd_dir("field", v, 1e-6)
+    ad = float(g["field"] @ v)
+    say(f"    <dL/dfield, v>: adjoint {ad:.10e} fd(f64) {fd:.10e} rel {rel(ad, fd):.2e}")
+    assert rel(ad, fd) < tol
