@@ -5,8 +5,10 @@
 Default workload: C2, 256x128x128 overlapping sphere pack (porosity 0.4),
 Langmuir LDF, 10000 steps.  Prints ONE JSON line with MLUPS (all cells, solids
 included), the HBM roofline of the fused kernel (algorithmic bytes per cell
-update x cells / measured step time, against 8 TB/s) and the CPU oracle timed
-serially on a bounded sample of the same workload.
+update x cells / measured step time, against 8 TB/s), the adjoint/forward time
+ratio of the checkpointed reverse sweep on the differentiation workload (C4,
+128x64x64 partial bounce-back, 4000 steps, checkpoint every 200) and the CPU
+oracle timed serially on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -103,6 +105,57 @@ def stream_probe(gib=2, reps=10):
     return {"stream_probe": out}
 
 
+def adjoint_timing(workload="C4", prec="f32", k=200, steps=None):
+    """Time of the checkpointed reverse sweep (recompute + adjoint steps) against the plain forward
+    run of the same steps, on the differentiation workload (C4: 128x64x64 partial bounce-back)."""
+    import torch
+
+    from fluidmaxxer_amd import abi, workloads
+    from fluidmaxxer_amd.adjoint import AdjointRun
+
+    nsteps = steps or workloads.SPECS[workload]["steps"]
+    cfg = workloads.build(workload, steps=nsteps)
+    run = AdjointRun(cfg, nsteps, k, prec)
+    sim = run.sim
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record(sim.stream)
+        out = fn()
+        e1.record(sim.stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), out
+
+    # plain forward: the same steps with the outlet reduction, no checkpoints
+    def plain():
+        abi.call("fmx_coupled_init", sim._desc_p(), sim._bufs_p(), 0, float(cfg.phi0), sim._s())
+        sim.cur, sim.step_index = 0, 0
+        acc = sim.outlet_acc(nsteps)
+        sim.launch_only(nsteps, acc=acc)
+
+    plain()  # warm-up
+    t_plain, _ = timed(plain)
+    t_fwd, c = timed(run.forward)
+    w = 2.0 * c  # loss sum_s C_out(s)^2
+    run.backward(w)  # warm-up
+    t_fwd, c = timed(run.forward)
+    t_bwd, _ = timed(lambda: run.backward(w))
+    res = {
+        "workload": f"{workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, {nsteps} steps, "
+                    f"checkpoint every {k}, {prec}",
+        "forward_ms": round(t_plain, 3),
+        "forward_with_checkpoints_ms": round(t_fwd, 3),
+        "backward_ms": round(t_bwd, 3),
+        "ratio": round(t_bwd / t_plain, 3),
+        "ratio_incl_checkpointed_forward": round((t_fwd + t_bwd) / t_plain, 3),
+    }
+    del run
+    torch.cuda.empty_cache()
+    return res
+
+
 def traffic_from_profile(workload, prec):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload.lower()}_{prec}.json")
     if not os.path.exists(path):
@@ -127,6 +180,8 @@ def main():
                     help="first run one plain device-to-device copy of BYTES (HBM-counter calibration)")
     ap.add_argument("--stream-probe", action="store_true",
                     help="instead of the benchmark, measure plain streaming bandwidth (copy / read / write)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -142,6 +197,10 @@ def main():
 
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
+        return
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_timing(prec="f64" if args.prec == "f64" else "f32")}), flush=True)
         return
 
     if args.calibrate_copy:
@@ -224,6 +283,10 @@ def main():
             "kernel": "k_coupled (one launch per step)",
         },
     }
+    if not args.no_adjoint and args.prec in ("f32", "f64"):
+        del sim
+        torch.cuda.empty_cache()
+        line["adjoint"] = adjoint_timing(prec=args.prec)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
     print(json.dumps(line), flush=True)

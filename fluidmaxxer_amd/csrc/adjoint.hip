@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                                                  i64 widx, double inv_count) {
     using C = typename Storage<P>::C;
     int bx, by, bz;
-    block_coords(prm.xcd_remap, prm.gx, prm.gy, prm.gz, bx, by, bz);
+    block_coords(prm, bx, by, bz);
     const int x = bx * prm.block_x + int(threadIdx.x);
     const int y = by * prm.block_y + int(threadIdx.y);
     const int z = bz;

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     constexpr unsigned SB = sizeof(S);
     constexpr unsigned CB = sizeof(C);
     int bx, by, bz;
-    block_coords(prm.xcd_remap, prm.gx, prm.gy, prm.gz, bx, by, bz);
+    block_coords(prm, bx, by, bz);
     const int x = bx * prm.block_x + int(threadIdx.x);
     const int y = by * prm.block_y + int(threadIdx.y);
     const int z = bz;

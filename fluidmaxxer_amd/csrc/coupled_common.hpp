@@ -16,6 +16,7 @@ template <class C> struct CoupledParams {
     int gx, gy, gz;          // logical block grid
     int block_x, block_y;
     int xcd_remap;
+    int tile_y, tile_z;      // row tiles (0: plain y-then-z order); they divide gy and gz
     i64 n;
     C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
 };
@@ -41,16 +42,26 @@ template <int P> struct CoupledPtrs {
 
 // Logical block id -> (bx, by, bz).  With xcd_remap the hardware's round-robin
 // assignment of consecutive workgroups to the 8 XCDs is inverted, so each XCD
-// sweeps one contiguous slab of the domain and its L2 sees the neighbouring rows
-// its pulls touch.
-__device__ __forceinline__ void block_coords(int remap, int gx, int gy, int gz, int &bx, int &by, int &bz) {
-    const unsigned total = unsigned(gx) * unsigned(gy) * unsigned(gz);
+// sweeps one contiguous range of the domain and its L2 sees the neighbouring rows
+// its pulls touch.  With tiles (tile_y x tile_z rows, y fastest inside a tile and
+// tiles y-fastest), the rows at z +- 1 are tile_y blocks away instead of a whole
+// y-plane of blocks, so they are still in L2 when their neighbours pull from them.
+template <class C> __device__ __forceinline__ void block_coords(const CoupledParams<C> &prm, int &bx, int &by, int &bz) {
+    const unsigned total = unsigned(prm.gx) * unsigned(prm.gy) * unsigned(prm.gz);
     unsigned b = blockIdx.x;
-    if (remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
-    bx = int(b % unsigned(gx));
-    const unsigned r = b / unsigned(gx);
-    by = int(r % unsigned(gy));
-    bz = int(r / unsigned(gy));
+    if (prm.xcd_remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
+    bx = int(b % unsigned(prm.gx));
+    const unsigned r = b / unsigned(prm.gx);
+    if (prm.tile_y) {
+        const unsigned tsz = unsigned(prm.tile_y) * unsigned(prm.tile_z);
+        const unsigned t = r / tsz, w = r % tsz;
+        const unsigned nty = unsigned(prm.gy) / unsigned(prm.tile_y);
+        by = int((t % nty) * unsigned(prm.tile_y) + w % unsigned(prm.tile_y));
+        bz = int((t / nty) * unsigned(prm.tile_z) + w / unsigned(prm.tile_y));
+    } else {
+        by = int(r % unsigned(prm.gy));
+        bz = int(r / unsigned(prm.gy));
+    }
 }
 
 // Population plane q of a structure-of-arrays buffer (wave-uniform pointer) and a load /
@@ -96,7 +107,7 @@ template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy
 // One wave-multiple row of x per block: 256 x 1 for long rows, 128 x 2 / 64 x 4 for short ones,
 // so no lane idles when nx is 64 or 128.  FMX_BLOCK=AxB and FMX_XCD_REMAP=0|1 override.
 struct LaunchShape {
-    int bx = 256, by = 1, remap = 1;
+    int bx = 256, by = 1, remap = 1, ty = 0, tz = 0;
 };
 
 inline LaunchShape launch_shape(int nx) {
@@ -112,6 +123,13 @@ inline LaunchShape launch_shape(int nx) {
         }
     }
     if (const char *e = std::getenv("FMX_XCD_REMAP")) s.remap = std::atoi(e) != 0;
+    if (const char *e = std::getenv("FMX_TILE")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0) {
+            s.ty = a;
+            s.tz = b;
+        }
+    }
     return s;
 }
 
@@ -135,6 +153,11 @@ template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
     p.gy = (d.ny + ls.by - 1) / ls.by;
     p.gz = d.nz;
     p.xcd_remap = ls.remap;
+    // tiles only where they divide the block grid exactly
+    if (ls.ty > 0 && ls.tz > 0 && p.gy % ls.ty == 0 && p.gz % ls.tz == 0) {
+        p.tile_y = ls.ty;
+        p.tile_z = ls.tz;
+    }
     p.n = i64(d.nx) * d.ny * d.nz;
     C tau = C(d.tau);
     if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204
