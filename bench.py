@@ -99,6 +99,10 @@ def stream_probe(gib=2, reps=10):
     out["stream_read_GBps"] = round(nbytes / timed(
         lambda: abi.call("fmx_stream_probe", 1, sink.data_ptr(), a.data_ptr(), nbytes, s)) / 1e9, 1)
     out["torch_sum_read_GBps"] = round(nbytes / timed(lambda: a.sum()) / 1e9, 1)
+    for mode, esz in ((2, 2), (3, 4), (4, 8)):
+        moved = 2 * (nbytes // (26 * esz)) * 26 * esz
+        out[f"plane_copy_{8 * esz}bit_GBps"] = round(moved / timed(
+            lambda: abi.call("fmx_stream_probe", mode, b.data_ptr(), a.data_ptr(), nbytes, s)) / 1e9, 1)
     out["write_GBps"] = round(nbytes / timed(
         lambda: abi.call("fmx_fill_zero", b.data_ptr(), nbytes, torch.cuda.current_stream().cuda_stream)) / 1e9, 1)
     out["bytes"] = nbytes

@@ -36,7 +36,7 @@
 //     in place as state (read, then overwritten with this step's value).
 //   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
 //     into FMX_OUTLET_SLOTS FP64 accumulators per step.
-#include "coupled_common.hpp"
+#include "coupled_cell.hpp"
 
 namespace fmx {
 
@@ -138,81 +138,12 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         q_old = p.q[cell];
     }
 
-    // ----------------------------------------------------- NS collide (in place)
-    if (!frozen) {
-        C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            rho += fo[q];
-            jx = axpy1<Q19::cx[q]>(jx, fo[q]);
-            jy = axpy1<Q19::cy[q]>(jy, fo[q]);
-            jz = axpy1<Q19::cz[q]>(jz, fo[q]);
-        });
-        C ux, uy, uz;
-        if (OP == FMX_NS_POROUS) {  // PorousBGKCollision.cu:118-136
-            if (!(rho > C(0))) {
-                rho = prm.rho0;
-                jx = jy = jz = C(0);
-            }
-            const C invR = C(1) / rho;
-            ux = jx * invR;
-            uy = jy * invR;
-            uz = jz * invR;
-            C ph = p.field ? p.field[cell] : C(1);
-            ph = ph < C(0) ? C(0) : (ph > C(1) ? C(1) : ph);
-            ux *= ph;
-            uy *= ph;
-            uz *= ph;
-        } else {  // BGKCollision.cu:149-158
-            if (!(rho > C(0))) {
-                rho = prm.rho0;
-                ux = uy = uz = C(0);
-            } else {
-                const C invR = C(1) / rho;
-                ux = jx * invR;
-                uy = jy * invR;
-                uz = jz * invR;
-            }
-        }
-        const C u2x15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
-        C ns = C(0);
-        if (OP == FMX_NS_PBB) {
-            ns = p.field ? p.field[cell] : C(0);
-            ns = ns < C(0) ? C(0) : (ns > C(1) ? C(1) : ns);
-        }
-        C fc[19];
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const C feq = feq_bgk<q>(rho, ux, uy, uz, u2x15);
-            fc[q] = fo[q] + prm.omega * (feq - fo[q]);
-        });
-        if (OP == FMX_NS_PBB) {
-            static_for<0, 19>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                fc[q] = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
-            });
-        }
-        if (!solid) {
-#pragma unroll
-            for (int q = 0; q < 19; ++q) fo[q] = fc[q];
-        }
-    }
-
-    // ------------------------------------- post-collision moments (Macros.cu:44-60)
+    // ------------------------------------------ NS collide (in place) and moments
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
     if (!frozen) {
-        C vx = C(0), vy = C(0), vz = C(0);
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            r2 += fo[q];
-            vx = axpy1<Q19::cx[q]>(vx, fo[q]);
-            vy = axpy1<Q19::cy[q]>(vy, fo[q]);
-            vz = axpy1<Q19::cz[q]>(vz, fo[q]);
-        });
-        const C inv2 = (r2 > C(0)) ? (C(1) / r2) : C(0);
-        ux2 = vx * inv2;
-        uy2 = vy * inv2;
-        uz2 = vz * inv2;
+        const C fld = p.field ? p.field[cell] : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
+        ns_collide_cell<OP>(fo, solid, fld, prm);
+        post_moments(fo, r2, ux2, uy2, uz2);
     }
 
     // ------------------------------------------------------------- NS stores
@@ -225,11 +156,9 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
             for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
         }
         if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
-            const C b15 = C(1.5) * prm.uin2;
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const C v = feq_bgk<q>(prm.rho0, prm.uin_x, prm.uin_y, prm.uin_z, b15);
-                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
             });
         }
     } else if (!outlet_cell && !frozen) {
@@ -237,59 +166,32 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
         for (int q = 0; q < 19; ++q)
             st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
     }
-    if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z) (VelocityDirichletNEQ.cu:81-95)
-        const C u2N = C(1.5) * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2);
-        const C u2B = C(1.5) * prm.uin2;
-        const unsigned dst_b = cell_b - SB;
+    if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const C feqN = feq_bgk<q>(r2, ux2, uy2, uz2, u2N);
-            const C feqB = feq_bgk<q>(r2, prm.uin_x, prm.uin_y, prm.uin_z, u2B);
-            st_b(plane(p.fn, q, n), dst_b, encode<P>(feqB + (fo[q] - feqN), C(Q19::inv_w[q]), prm.f_ref));
+            st_b(plane(p.fn, q, n), cell_b - SB,
+                 encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
         });
     }
-    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z) (PressureDirichlet.cu:81-95)
-        const C u2o = C(1.5) * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2);
-        const unsigned dst_b = cell_b + SB;
+    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux2, uy2, uz2);
-            const C v = w19<C>(q) * prm.rho_out * (C(1) + C(3) * eu + C(4.5) * eu * eu - u2o);
-            st_b(plane(p.fn, q, n), dst_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+            st_b(plane(p.fn, q, n), cell_b + SB,
+                 encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
         });
     }
 
-    // ------------------------------------------------------ sorption source
+    // ------------------------------------------------------ sorption source and AD collision
     C src_term = C(0);
-    if (sorbs) {  // LinearLDF.cu:27-47, LangmuirLDF.cu:23-44
-        C c = phi_lag;
-        if (c < prm.min_conc) c = prm.min_conc;
-        const C ce = c - prm.min_conc;
-        C qeq;
-        if (prm.isotherm == FMX_ISO_LANGMUIR) {
-            const C bc = prm.iso_b * ce;
-            qeq = prm.iso_a * (bc / (C(1) + bc));
-        } else {
-            qeq = prm.iso_a * ce;
-        }
-        const C dq = prm.k_ldf * (qeq - q_old);
-        p.q_new[cell] = q_old + dq;
-        src_term = -dq;
+    if (sorbs) {
+        C q_new;
+        src_term = ldf_cell(phi_lag, q_old, prm, q_new);
+        p.q_new[cell] = q_new;
     }
-
-    C phi = C(0);
-#pragma unroll
-    for (int q = 0; q < 7; ++q) phi += gi[q];
-    if (phi < prm.min_conc) phi = prm.min_conc;
     const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
     const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
     C go[7];
-    static_for<0, 7>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const C eu = dot3<Q7::cx[q], Q7::cy[q], Q7::cz[q]>(ax, ay, az);
-        const C geq = w7<C>(q) * phi * (C(1) + C(4) * eu);
-        go[q] = gi[q] + om * (geq - gi[q]) + w7<C>(q) * src_term;
-    });
+    const C phi = ad_collide_cell(gi, src_term, om, ax, ay, az, prm.min_conc, go);
     p.phi_new[cell] = phi;
 
     // ------------------------------------------------------------- AD stores
@@ -533,6 +435,9 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
 template <int P>
 int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
                         i64 step, i64 acc_row, hipStream_t s) {
+    if constexpr (P != FMX_F64) {
+        if (prm.pair) return launch_coupled_pair<P>(ns_op, ptr, prm, step, acc_row, s);
+    }
     const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
     // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads (measured slower; kept for study)
     static const bool nt_loads = [] {

@@ -17,6 +17,7 @@ template <class C> struct CoupledParams {
     int block_x, block_y;
     int xcd_remap;
     int tile_y, tile_z;      // row tiles (0: plain y-then-z order); they divide gy and gz
+    int pair;                // two cells per thread (k_coupled_pair) where the storage and nx allow it
     i64 n;
     C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
 };
@@ -158,6 +159,12 @@ template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
         p.tile_y = ls.ty;
         p.tile_z = ls.tz;
     }
+    // FMX_PAIR=1 selects the two-cells-per-thread kernel where nx % 128 == 0 (measured no faster for
+    // FP32 and slower for FP16/BF16 storage, so off by default); the frozen-solid path (FMX_FREEZE=1)
+    // runs only in the one-cell-per-thread kernel
+    const char *pe = std::getenv("FMX_PAIR");
+    const char *fz = std::getenv("FMX_FREEZE");
+    p.pair = (pe && std::atoi(pe) != 0) && !(fz && std::atoi(fz) != 0) && d.nx >= 128 && d.nx % 128 == 0;
     p.n = i64(d.nx) * d.ny * d.nz;
     C tau = C(d.tau);
     if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204
@@ -183,6 +190,10 @@ inline unsigned coupled_blocks(int gx, int gy, int gz) { return unsigned(gx) * u
 // The fused forward step (coupled.hip): one launch over the whole box.
 template <int P>
 int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
+                        i64 step, i64 acc_row, hipStream_t s);
+// The same step with two cells per thread (coupled_pair.hip); FP32 / FP16 / BF16 storage, nx % 128 == 0.
+template <int P>
+int launch_coupled_pair(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
                         i64 step, i64 acc_row, hipStream_t s);
 // nsteps fused steps on the ping-pong buffers of b, outlet sums of step step0+k into row acc_row0+k.
 template <int P>

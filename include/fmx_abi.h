@@ -266,8 +266,10 @@ int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *b
 int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
                     void *stream);
 
-/* Streaming-bandwidth probes (16 B per lane, non-temporal): mode 0 copies `bytes` from src to dst,
- * mode 1 only reads src (dst: a 4-byte sink).  bytes must be a multiple of 16. */
+/* Streaming-bandwidth probes: mode 0 copies `bytes` from src to dst (16 B per lane, non-temporal),
+ * mode 1 only reads src (dst: a 4-byte sink); modes 2 / 3 / 4 copy 26 planes of n = bytes / (26 * e)
+ * elements of e = 2 / 4 / 8 bytes, one element per plane per thread (the population access
+ * pattern of the fused step without its arithmetic).  bytes must be a multiple of 16. */
 int fmx_stream_probe(int mode, void *dst, const void *src, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------ geometry */
