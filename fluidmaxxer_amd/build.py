@@ -47,9 +47,16 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
+# FP16 / BF16 storage kernels: FMA contraction allowed (their accuracy is set by the 16-bit storage)
+CONTRACT_FAST = {"coupled_half.hip"}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    flags = CFLAGS
+    if os.path.basename(src) in CONTRACT_FAST:
+        flags = ["-ffp-contract=fast" if f == "-ffp-contract=off" else f for f in CFLAGS]
+    cmd = [HIPCC, *flags, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -75,21 +82,38 @@ def build(jobs: int = 0) -> str:
     return LIB
 
 
+def _gxx(src: str, exe: str, dt: str) -> str:
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-DFMX_DATA_TYPE={dt}", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(PKG, 'host')}", src, "-o", exe, f"-L{LIBDIR}", "-lfmx_hip", f"-Wl,-rpath,{LIBDIR}",
+           f"-Wl,-rpath,$ORIGIN/{os.path.relpath(LIBDIR, os.path.dirname(exe))}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+    return exe
+
+
 def build_cpp_driver() -> list:
-    """The C++ API driver of the test suite (tests/cpp/coupled_driver.cpp), float and double."""
-    src = os.path.join(ROOT, "tests", "cpp", "coupled_driver.cpp")
+    """The C++ API programs of the test suite (tests/cpp/*.cpp), float and double."""
     out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
     os.makedirs(out_dir, exist_ok=True)
     outs = []
-    for dt in ("float", "double"):
-        exe = os.path.join(out_dir, f"coupled_driver_{dt}")
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-DFMX_DATA_TYPE={dt}", f"-I{os.path.join(ROOT, 'include')}",
-               f"-I{os.path.join(PKG, 'host')}", src, "-o", exe, f"-L{LIBDIR}", "-lfmx_hip", f"-Wl,-rpath,{LIBDIR}",
-               "-Wl,-rpath,$ORIGIN/../../../fluidmaxxer_amd/lib"]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"driver build failed:\n{r.stderr}")
-        outs.append(exe)
+    for name in ("coupled_driver", "host_units"):
+        src = os.path.join(ROOT, "tests", "cpp", f"{name}.cpp")
+        for dt in ("float", "double"):
+            outs.append(_gxx(src, os.path.join(out_dir, f"{name}_{dt}"), dt))
+    return outs
+
+
+def build_apps() -> list:
+    """The main.cpp-equivalent driver (fluidmaxxer_amd/host/apps/lbm_coupled.cpp), float and double."""
+    src_dir = os.path.join(PKG, "host", "apps")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for src in sorted(glob.glob(os.path.join(src_dir, "*.cpp"))):
+        base = os.path.splitext(os.path.basename(src))[0]
+        for dt in ("float", "double"):
+            outs.append(_gxx(src, os.path.join(out_dir, f"{base}_{dt}"), dt))
     return outs
 
 
@@ -99,3 +123,4 @@ if __name__ == "__main__":
     a = ap.parse_args()
     print(build(a.jobs))
     print(*build_cpp_driver(), sep="\n")
+    print(*build_apps(), sep="\n")

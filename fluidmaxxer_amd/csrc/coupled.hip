@@ -36,188 +36,9 @@
 //     in place as state (read, then overwritten with this step's value).
 //   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
 //     into FMX_OUTLET_SLOTS FP64 accumulators per step.
-#include "coupled_cell.hpp"
+#include "coupled_kernel.hpp"
 
 namespace fmx {
-
-template <int P, int OP, bool NTL>
-__global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
-                                                 i64 acc_row) {
-    using C = typename Storage<P>::C;
-    using S = typename Storage<P>::S;
-    constexpr unsigned SB = sizeof(S);
-    constexpr unsigned CB = sizeof(C);
-    int bx, by, bz;
-    block_coords(prm, bx, by, bz);
-    const int x = bx * prm.block_x + int(threadIdx.x);
-    const int y = by * prm.block_y + int(threadIdx.y);
-    const int z = bz;
-    if (x >= prm.nx || y >= prm.ny) return;
-    const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
-    const i64 n = prm.n;
-    const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
-    const unsigned cell_b = cell * SB;
-    const std::uint32_t lk = p.links[cell];
-    const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
-    // a frozen solid cell alternates between the two states held in f and f_new and has zero
-    // post-collision velocity in both: no NS loads, collision or stores
-    const bool frozen = (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
-
-    // neighbour rows: x keeps local out of domain, y/z wrap when periodic
-    const bool xm_ok = x > 0, xp_ok = x < nx - 1;
-    int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
-    bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
-    if (prm.py) {
-        if (!ym_ok) ym += ny;
-        if (!yp_ok) yp -= ny;
-        ym_ok = yp_ok = true;
-    }
-    if (prm.pz) {
-        if (!zm_ok) zm += nz;
-        if (!zp_ok) zp -= nz;
-        zm_ok = zp_ok = true;
-    }
-    const unsigned ys[3] = {unsigned(ym), unsigned(y), unsigned(yp)};
-    const unsigned zs[3] = {unsigned(zm), unsigned(z), unsigned(zp)};
-    const bool yok[3] = {ym_ok, true, yp_ok};
-    const bool zok[3] = {zm_ok, true, zp_ok};
-    const bool xok[3] = {xm_ok, true, xp_ok};
-    // source cell of the pull along (cx, cy, cz) = cell - c, or the cell itself when out of domain
-    auto src_of = [&](int cx, int cy, int cz) -> unsigned {
-        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
-        return ok ? unsigned(x - cx) + unsigned(nx) * (ys[1 - cy] + unsigned(ny) * zs[1 - cz]) : cell;
-    };
-
-    // ---------------------------------------------------------------- NS pull
-    C fo[19];
-    if (!frozen) static_for<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
-        const unsigned src = src_of(cx, cy, cz);
-        bool bb = false;
-        if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
-        S raw;
-        if constexpr (NTL) {
-            if (bb) raw = ld_b_nt(plane(p.f, Q19::opp[q], n), cell_b);
-            else raw = ld_b_nt(plane(p.f, q, n), src * SB);
-        } else {
-            if (bb) raw = ld_b(plane(p.f, Q19::opp[q], n), cell_b);
-            else raw = ld_b(plane(p.f, q, n), src * SB);
-        }
-        fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
-    });
-
-    // ------------------------------------------- AD pull and sorption inputs
-    // (issued with the NS pulls; nothing below writes what they read)
-    const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
-    C gi[7];
-    static_for<0, 7>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-        const unsigned s = src_of(cx, cy, cz);
-        // x of the source (or keep-local) cell: x - cx when in the domain, else x
-        const int sx = (cx == 1) ? (x > 0 ? x - 1 : x) : ((cx == -1) ? (x < nx - 1 ? x + 1 : x) : x);
-        if (prm.ad_inlet == 2 && sx == 0) {
-            gi[q] = phi_pre * w7<C>(q);
-        } else {
-            const S raw = NTL ? ld_b_nt(plane(p.g, q, n), s * SB) : ld_b(plane(p.g, q, n), s * SB);
-            if constexpr (IsHalf<P>::value) {
-                // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
-                // populations of its x - 1 neighbour, stored about that neighbour's concentration
-                const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                gi[q] = w7<C>(q) * (p.phi[sref] + C(raw));
-            } else {
-                gi[q] = raw;
-            }
-        }
-    });
-    const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
-    C phi_lag = C(0), q_old = C(0);
-    if (sorbs) {
-        phi_lag = p.phi[cell];
-        q_old = p.q[cell];
-    }
-
-    // ------------------------------------------ NS collide (in place) and moments
-    C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
-    if (!frozen) {
-        const C fld = p.field ? p.field[cell] : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
-        ns_collide_cell<OP>(fo, solid, fld, prm);
-        post_moments(fo, r2, ux2, uy2, uz2);
-    }
-
-    // ------------------------------------------------------------- NS stores
-    const bool inlet_cell = prm.ns_inlet && x == 0;
-    const bool outlet_cell = prm.ns_outlet && x == nx - 1;
-    if (inlet_cell) {
-        const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
-        if (p.inlet_raw) {
-#pragma unroll
-            for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
-        }
-        if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
-            static_for<0, 19>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
-            });
-        }
-    } else if (!outlet_cell && !frozen) {
-#pragma unroll
-        for (int q = 0; q < 19; ++q)
-            st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
-    }
-    if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            st_b(plane(p.fn, q, n), cell_b - SB,
-                 encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
-        });
-    }
-    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            st_b(plane(p.fn, q, n), cell_b + SB,
-                 encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
-        });
-    }
-
-    // ------------------------------------------------------ sorption source and AD collision
-    C src_term = C(0);
-    if (sorbs) {
-        C q_new;
-        src_term = ldf_cell(phi_lag, q_old, prm, q_new);
-        p.q_new[cell] = q_new;
-    }
-    const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
-    const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
-    C go[7];
-    const C phi = ad_collide_cell(gi, src_term, om, ax, ay, az, prm.min_conc, go);
-    p.phi_new[cell] = phi;
-
-    // ------------------------------------------------------------- AD stores
-    // (half storage: deviation about this cell's new concentration, which the readers load as phi)
-    const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
-    if (prm.ad_inlet == 1 && x == 0) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q)
-            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
-    } else if (!ad_out_cell) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
-    }
-    if (prm.ad_outlet && x == nx - 2) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), phi));
-    }
-
-    // ------------------------------------------------- outlet breakthrough sum
-    if (p.outlet_acc && x == nx - 1) {
-        const int slot = int((unsigned(y) + unsigned(ny) * unsigned(z)) % FMX_OUTLET_SLOTS);
-        unsafeAtomicAdd(p.outlet_acc + acc_row * (FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE) + slot * FMX_OUTLET_STRIDE,
-                        double(phi));
-    }
-    (void)CB;
-}
 
 // Link words: bounce-back bits of the D3Q19 pulls (BGKCollision.cu:90-97, 124-126), region, solid.
 __global__ void __launch_bounds__(256) k_build_links(const std::uint8_t *__restrict__ flags,
@@ -432,42 +253,16 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     return p;
 }
 
-template <int P>
-int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
-                        i64 step, i64 acc_row, hipStream_t s) {
-    if constexpr (P != FMX_F64) {
-        if (prm.pair) return launch_coupled_pair<P>(ns_op, ptr, prm, step, acc_row, s);
-    }
-    const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
-    // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads (measured slower; kept for study)
-    static const bool nt_loads = [] {
-        const char *e = std::getenv("FMX_NT_LOADS");
-        return e != nullptr && std::atoi(e) != 0;
-    }();
-#define FMX_LAUNCH(OP, NT) hipLaunchKernelGGL((k_coupled<P, OP, NT>), grid, block, 0, s, ptr, prm, step, acc_row)
-    switch (ns_op) {
-        case FMX_NS_BGK:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_BGK, true);
-            else FMX_LAUNCH(FMX_NS_BGK, false);
-            break;
-        case FMX_NS_POROUS:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_POROUS, true);
-            else FMX_LAUNCH(FMX_NS_POROUS, false);
-            break;
-        case FMX_NS_PBB:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_PBB, true);
-            else FMX_LAUNCH(FMX_NS_PBB, false);
-            break;
-        default: return set_error(FMX_ERR_ARG, "fmx_coupled: unknown ns_op %d", ns_op);
-    }
-#undef FMX_LAUNCH
-    return after_launch("k_coupled");
-}
-
 template int launch_coupled_step<FMX_F32>(int, const CoupledPtrs<FMX_F32> &, const CoupledParams<float> &, i64, i64,
                                           hipStream_t);
 template int launch_coupled_step<FMX_F64>(int, const CoupledPtrs<FMX_F64> &, const CoupledParams<double> &, i64, i64,
                                           hipStream_t);
+
+// FP16 / BF16 storage: instantiated in coupled_half.hip only
+extern template int launch_coupled_step<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64,
+                                                 i64, hipStream_t);
+extern template int launch_coupled_step<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64,
+                                                  i64, hipStream_t);
 
 // Steps step0 .. step0+nsteps-1 on the ping-pong buffers; the outlet sums of step step0+k go to row
 // acc_row0 + k of outlet_acc.

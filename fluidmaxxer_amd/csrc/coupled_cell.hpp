@@ -1,12 +1,22 @@
 // coupled_cell.hpp -- the per-cell arithmetic of the fused coupled step, shared by the
-// one-cell-per-thread and the two-cells-per-thread kernels.  Each function evaluates the
-// reference's expressions in the reference's order (no FMA contraction in this build), so
-// every kernel that uses them gives the same bits.
+// one-cell-per-thread and the two-cells-per-thread kernels and by every storage precision.  Each
+// function evaluates the reference's expressions in the reference's order; compiled without FMA
+// contraction and with FAST = false (FP32 / FP64 storage) every kernel gives the same bits as the
+// serial restatement of the reference.  FAST (FP16 / BF16 storage) swaps IEEE division for the
+// hardware reciprocal.
 #pragma once
+
+#include <type_traits>
 
 #include "coupled_common.hpp"
 
 namespace fmx {
+
+// 1/x: IEEE division, or with FAST the hardware reciprocal (v_rcp_f32, 1 ulp) for float.
+template <bool FAST, class C> __device__ __forceinline__ C recip(C x) {
+    if constexpr (FAST && std::is_same_v<C, float>) return __builtin_amdgcn_rcpf(x);
+    else return C(1) / x;
+}
 
 // NS collision of one cell, in place: fo holds the pulled populations on entry and the
 // populations the step stores on exit (unchanged for a BGK solid cell).
@@ -14,7 +24,7 @@ namespace fmx {
 //   porous BGK:   PorousBGKCollision.cu:118-150, u scaled by the clamped porosity `fld`
 //   partial BB:   BGK collision blended with the reversed pulled populations by the clamped
 //                 solid fraction `fld` (Walsh et al. 2009)
-template <int OP, class C>
+template <int OP, bool FAST = false, class C>
 __device__ __forceinline__ void ns_collide_cell(C (&fo)[19], bool solid, C fld, const CoupledParams<C> &prm) {
     C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
     static_for<0, 19>([&](auto qc) {
@@ -30,7 +40,7 @@ __device__ __forceinline__ void ns_collide_cell(C (&fo)[19], bool solid, C fld, 
             rho = prm.rho0;
             jx = jy = jz = C(0);
         }
-        const C invR = C(1) / rho;
+        const C invR = recip<FAST>(rho);
         ux = jx * invR;
         uy = jy * invR;
         uz = jz * invR;
@@ -43,7 +53,7 @@ __device__ __forceinline__ void ns_collide_cell(C (&fo)[19], bool solid, C fld, 
             rho = prm.rho0;
             ux = uy = uz = C(0);
         } else {
-            const C invR = C(1) / rho;
+            const C invR = recip<FAST>(rho);
             ux = jx * invR;
             uy = jy * invR;
             uz = jz * invR;
@@ -71,7 +81,7 @@ __device__ __forceinline__ void ns_collide_cell(C (&fo)[19], bool solid, C fld, 
 }
 
 // Velocity of the stored populations (Macros.cu:44-60): the AD coupling and the boundary updates.
-template <class C>
+template <bool FAST = false, class C>
 __device__ __forceinline__ void post_moments(const C (&fo)[19], C &r2, C &ux2, C &uy2, C &uz2) {
     C r = C(0), vx = C(0), vy = C(0), vz = C(0);
     static_for<0, 19>([&](auto qc) {
@@ -81,7 +91,7 @@ __device__ __forceinline__ void post_moments(const C (&fo)[19], C &r2, C &ux2, C
         vy = axpy1<Q19::cy[q]>(vy, fo[q]);
         vz = axpy1<Q19::cz[q]>(vz, fo[q]);
     });
-    const C inv2 = (r > C(0)) ? (C(1) / r) : C(0);
+    const C inv2 = (r > C(0)) ? recip<FAST>(r) : C(0);
     r2 = r;
     ux2 = vx * inv2;
     uy2 = vy * inv2;
@@ -113,14 +123,15 @@ template <int q, class C> __device__ __forceinline__ C outlet_pressure(C ux2, C 
 
 // LDF sorption of one region cell (LinearLDF.cu:27-47, LangmuirLDF.cu:23-44): the new adsorbed load
 // and the source term S = -dq, from the previous step's concentration.
-template <class C> __device__ __forceinline__ C ldf_cell(C phi_lag, C q_old, const CoupledParams<C> &prm, C &q_new) {
+template <bool FAST = false, class C>
+__device__ __forceinline__ C ldf_cell(C phi_lag, C q_old, const CoupledParams<C> &prm, C &q_new) {
     C c = phi_lag;
     if (c < prm.min_conc) c = prm.min_conc;
     const C ce = c - prm.min_conc;
     C qeq;
     if (prm.isotherm == FMX_ISO_LANGMUIR) {
         const C bc = prm.iso_b * ce;
-        qeq = prm.iso_a * (bc / (C(1) + bc));
+        qeq = prm.iso_a * (FAST ? bc * recip<FAST>(C(1) + bc) : bc / (C(1) + bc));
     } else {
         qeq = prm.iso_a * ce;
     }

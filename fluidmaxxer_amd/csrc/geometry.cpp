@@ -62,6 +62,63 @@ struct Sphere {
 
 using namespace fmx;
 
+namespace {
+
+// RFC 4648 base64 (whitespace ignored, stops at padding)
+std::string base64_decode(const std::string &in) {
+    auto val = [](char c) -> int {
+        if (c >= 'A' && c <= 'Z') return c - 'A';
+        if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+        if (c >= '0' && c <= '9') return c - '0' + 52;
+        if (c == '+') return 62;
+        if (c == '/') return 63;
+        return -1;
+    };
+    std::string out;
+    unsigned acc = 0;
+    int bits = 0;
+    for (char c : in) {
+        if (c == '=') break;
+        const int v = val(c);
+        if (v < 0) continue;
+        acc = (acc << 6) | unsigned(v);
+        bits += 6;
+        if (bits >= 8) {
+            bits -= 8;
+            out.push_back(char((acc >> bits) & 0xFFu));
+        }
+    }
+    return out;
+}
+
+// little-endian array of a VTK scalar type -> doubles
+bool decode_values(const std::string &type, const char *p, size_t nbytes, std::vector<double> &vals) {
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        const size_t n = nbytes / sizeof(T);
+        vals.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            T v;
+            std::memcpy(&v, p + i * sizeof(T), sizeof(T));
+            vals[i] = double(v);
+        }
+        return true;
+    };
+    if (type == "UInt8") return run(std::uint8_t{});
+    if (type == "Int8") return run(std::int8_t{});
+    if (type == "UInt16") return run(std::uint16_t{});
+    if (type == "Int16") return run(std::int16_t{});
+    if (type == "UInt32") return run(std::uint32_t{});
+    if (type == "Int32") return run(std::int32_t{});
+    if (type == "UInt64") return run(std::uint64_t{});
+    if (type == "Int64") return run(std::int64_t{});
+    if (type == "Float32") return run(float{});
+    if (type == "Float64") return run(double{});
+    return false;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fmx_geom_sphere_pack(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double r_min, double r_max,
@@ -236,21 +293,63 @@ int fmx_vti_read_mask(const char *path, double threshold, int32_t *dims, uint8_t
     if (da == std::string::npos) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, no <DataArray> found in VTI File.");
     const size_t gt = content.find('>', da);
     if (gt == std::string::npos) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, invalid DataArray-Tag.");
-    const size_t end = content.find("</DataArray>", gt);
-    if (end == std::string::npos)
-        return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, could not found closing </DataArray>-Tag!");
-    // parse numbers in content[gt+1, end) until the first non-number
-    std::string body = content.substr(gt + 1, end - gt - 1);
+    const std::string tag = content.substr(da, gt - da + 1);
+    auto tag_attr = [&](const char *nm) -> std::string {
+        const std::string pat = std::string(" ") + nm + "=\"";
+        size_t q = tag.find(pat);
+        if (q == std::string::npos) return {};
+        q += pat.size();
+        const size_t e = tag.find('"', q);
+        return e == std::string::npos ? std::string() : tag.substr(q, e - q);
+    };
+    const std::string format = tag_attr("format");
     std::vector<double> vals;
-    vals.reserve(size_t(N));
-    const char *p = body.c_str();
-    for (;;) {
-        char *q = nullptr;
-        errno = 0;
-        const double v = std::strtod(p, &q);
-        if (q == p) break;
-        vals.push_back(v);
-        p = q;
+    if (format == "appended" || format == "binary") {
+        // raw appended or inline base64 binary arrays (extension: the reference reads ASCII only)
+        if (!attr("compressor").empty())
+            return set_error(FMX_ERR_UNSUPPORTED, "[VTIReader]: compressed binary arrays are not supported");
+        const size_t hsize = attr("header_type") == "UInt64" ? 8 : 4;
+        std::string blob;
+        if (format == "appended") {
+            const std::string off = tag_attr("offset");
+            const size_t ad = content.find("<AppendedData");
+            const size_t us = ad == std::string::npos ? ad : content.find('_', content.find('>', ad));
+            if (off.empty() || us == std::string::npos)
+                return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, appended DataArray without offset or AppendedData.");
+            const std::string enc = content.substr(ad, content.find('>', ad) - ad);
+            if (enc.find("encoding=\"raw\"") == std::string::npos)
+                return set_error(FMX_ERR_UNSUPPORTED, "[VTIReader]: only raw-encoded AppendedData is supported");
+            const size_t at = us + 1 + size_t(std::stoull(off));
+            if (at + hsize > content.size()) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, appended offset out of range.");
+            blob = content.substr(at);
+        } else {
+            const size_t end = content.find("</DataArray>", gt);
+            if (end == std::string::npos)
+                return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, could not found closing </DataArray>-Tag!");
+            blob = base64_decode(content.substr(gt + 1, end - gt - 1));
+        }
+        if (blob.size() < hsize) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, truncated binary array.");
+        std::uint64_t nbytes = 0;
+        std::memcpy(&nbytes, blob.data(), hsize);
+        if (hsize + nbytes > blob.size()) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, truncated binary array.");
+        if (!decode_values(tag_attr("type"), blob.data() + hsize, size_t(nbytes), vals))
+            return set_error(FMX_ERR_UNSUPPORTED, "[VTIReader]: unsupported DataArray type '%s'", tag_attr("type").c_str());
+    } else {
+        const size_t end = content.find("</DataArray>", gt);
+        if (end == std::string::npos)
+            return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, could not found closing </DataArray>-Tag!");
+        // parse numbers in content[gt+1, end) until the first non-number
+        std::string body = content.substr(gt + 1, end - gt - 1);
+        vals.reserve(size_t(N));
+        const char *p = body.c_str();
+        for (;;) {
+            char *q = nullptr;
+            errno = 0;
+            const double v = std::strtod(p, &q);
+            if (q == p) break;
+            vals.push_back(v);
+            p = q;
+        }
     }
     if (vals.empty()) return set_error(FMX_ERR_ARG, "[VTIReader]: Fatal, DataArray does not contain numerical values!");
     std::memset(mask, 0, size_t(N));
