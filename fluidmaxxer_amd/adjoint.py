@@ -34,8 +34,8 @@ def _torch():
 class AdjointRun:
     def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0"):
         torch = _torch()
-        if prec not in ("f32", "f64"):
-            raise abi.FmxError("the adjoint runs on f32 or f64 state")
+        if prec not in ("f32", "f64", "f16", "bf16"):
+            raise abi.FmxError(f"unknown precision {prec}")
         if cfg.ns_inlet == "neq":
             raise abi.FmxError("the NEQ velocity inlet has no adjoint; use ns_inlet='eq'")
         if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):

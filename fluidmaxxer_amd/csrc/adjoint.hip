@@ -15,12 +15,13 @@
 // then back-propagates through its local update and writes the adjoint of what it pulled.
 // No atomics are needed for the populations and every adjoint slot is written once per step.
 //
-// Boundary rules (mirroring how k_coupled folds the boundary conditions into the step):
+// Boundary rules (the boundary conditions as the fused step folds them in):
 //   * x = 0 with an equilibrium NS inlet, or an ADDirichlet AD inlet: the stored values are
 //     constants, so the owner's adjoint of them is zero.
-//   * x = nx-1 with a pressure outlet / Neumann outlet: the outlet cell stores nothing; the
-//     thread of (nx-2, y, z), which wrote the outlet slots, gathers their adjoint and
-//     back-propagates it through the pressure equilibrium / the copied g.
+//   * x = nx-1 with a pressure outlet / Neumann outlet: the outlet slots hold functions of the
+//     (nx-2, y, z) cell's post-collision state (the pressure equilibrium of its velocity, a copy
+//     of its g), so that cell's reverse step gathers their adjoint and back-propagates it; the
+//     outlet cell's own collision result is discarded and gets none.
 //   * per-step inlet series: pulls whose source lies on x = 0 read the series value; their
 //     adjoint accumulates into grad_series[step].
 //   * clamps (phi >= min_conc, field in [0, 1], tau >= 0.5001) differentiate the active branch;
@@ -127,8 +128,9 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
             bool bb = false;
             if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
-            if (bb) fo[q] = p.f[i64(Q19::opp[q]) * n + cell];
-            else fo[q] = p.f[i64(q) * n + (inside(x - cx, -cy, -cz) ? at(x - cx, -cy, -cz) : cell)];
+            const auto raw = bb ? p.f[i64(Q19::opp[q]) * n + cell]
+                                : p.f[i64(q) * n + (inside(x - cx, -cy, -cz) ? at(x - cx, -cy, -cz) : cell)];
+            fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
         });
         const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
         C gi[7];
@@ -139,7 +141,19 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             const bool in = inside(x - cx, -cy, -cz);
             const int sx = in ? x - cx : x;
             subst[q] = prm.ad_inlet == 2 && sx == 0;
-            gi[q] = subst[q] ? phi_pre * w7<C>(q) : p.g[i64(q) * n + (in ? at(x - cx, -cy, -cz) : cell)];
+            if (subst[q]) {
+                gi[q] = phi_pre * w7<C>(q);
+            } else {
+                const unsigned s = in ? at(x - cx, -cy, -cz) : cell;
+                if constexpr (IsHalf<P>::value) {
+                    // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
+                    // on a Neumann outlet plane), as the forward step stores them
+                    const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
+                    gi[q] = w7<C>(q) * (p.phi[sref] + C(p.g[i64(q) * n + s]));
+                } else {
+                    gi[q] = p.g[i64(q) * n + s];
+                }
+            }
         });
         const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
         const C phi_lag = sorbs ? p.phi[cell] : C(0);
@@ -539,8 +553,7 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
 static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
                          int32_t k, const char *who) {
     if (int rc = fmx_coupled_validate(d, b)) return rc;
-    if (d->prec != FMX_F32 && d->prec != FMX_F64)
-        return set_error(FMX_ERR_UNSUPPORTED, "%s: the adjoint runs on FP32 or FP64 state", who);
+
     if (d->ns_inlet == 1)
         return set_error(FMX_ERR_UNSUPPORTED, "%s: the NEQ velocity inlet has no adjoint (use the equilibrium inlet)",
                          who);
@@ -614,6 +627,8 @@ int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *d) {
     switch (d->prec) {
         case FMX_F32: return state_bytes<FMX_F32>(n);
         case FMX_F64: return state_bytes<FMX_F64>(n);
+        case FMX_F16: return state_bytes<FMX_F16>(n);
+        case FMX_BF16: return state_bytes<FMX_BF16>(n);
         default: return -1;
     }
 }
@@ -624,8 +639,12 @@ int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, in
     if (int rc = adjoint_check(d, b, step0, nsteps, k, "fmx_adjoint_forward")) return rc;
     if ((cur != 0 && cur != 1) || !ckpt) return set_error(FMX_ERR_ARG, "fmx_adjoint_forward: bad cur or checkpoint store");
     hipStream_t s = as_stream(stream);
-    if (d->prec == FMX_F32) return forward_ckpt<FMX_F32>(*d, *b, step0, nsteps, k, cur, ckpt, s);
-    return forward_ckpt<FMX_F64>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+    switch (d->prec) {
+        case FMX_F32: return forward_ckpt<FMX_F32>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+        case FMX_F64: return forward_ckpt<FMX_F64>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+        case FMX_F16: return forward_ckpt<FMX_F16>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+        default: return forward_ckpt<FMX_BF16>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+    }
     FMX_TRY_END
 }
 
@@ -642,8 +661,12 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: grad_series needs the per-step inlet series (ad_inlet 2)");
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
-    if (d->prec == FMX_F32) return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
-    return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+    switch (d->prec) {
+        case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+        case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+        case FMX_F16: return backward<FMX_F16>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+        default: return backward<FMX_BF16>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+    }
     FMX_TRY_END
 }
 

@@ -10,32 +10,30 @@
 //        -> PostStreaming BCs (ADDirichlet inlet, ADNeumannOutlet) -> swap
 //
 // How the boundary conditions fold into one launch without changing results:
-//   * Pressure outlet (x = nx-1): its populations are feq(rho_out, u) with u the
-//     moments of the post-collision cell at x = nx-2 (PressureDirichlet.cu:46-92).
-//     The thread of cell (nx-2, y, z) holds exactly those populations in
-//     registers, so it writes the outlet cell too; the outlet thread writes no f.
-//   * AD Neumann outlet: g_new(nx-1) = g_new(nx-2) (ADNeumannOutlet.cu:21-30): the
-//     thread of (nx-2, y, z) stores its g twice; the outlet thread stores only phi.
-//     The outlet cell's own AD collision result (and so its velocity) is never
-//     observable, which is why a pressure outlet requires the Neumann outlet.
+//   * Pressure outlet (x = nx-1): its populations are feq(rho_out, u) with u the moments of the
+//     post-collision cell at x = nx-2 (PressureDirichlet.cu:46-92).  The outlet cell's lane takes
+//     that velocity from the neighbouring lane (a lane shuffle; both cells lie in one wave) and
+//     stores the outlet populations itself, in the same full-wave store as every other cell.
+//   * AD Neumann outlet: g_new(nx-1) = g_new(nx-2) (ADNeumannOutlet.cu:21-30): the outlet lane
+//     stores its neighbour's g.  The outlet cell's own AD collision result (and so its velocity)
+//     is never observable, which is why a pressure outlet requires the Neumann outlet.
 //   * NEQ velocity inlet (x = 0): step s+1 starts by replacing f(0,y,z) with
-//     feq(rho_N, u_BC) + f_N - feq(rho_N, u_N), f_N the populations of (1,y,z) after
-//     step s (VelocityDirichletNEQ.cu:49-96).  The thread of (1,y,z) computes that
-//     at the end of step s and stores it into f_new(0,y,z); the x = 0 thread stores
-//     its own post-collision populations into the `inlet_raw` plane instead, which
-//     is what the reference holds in f(0,y,z) between steps (used for the AD
-//     velocity here and for state read-back).  fmx_coupled_prologue applies the
-//     first step's update.
-//   * Equilibrium velocity inlet (VelocityDirichletEq, VelocityDirichlet.cu:54-84):
-//     the x = 0 thread stores feq(rho0, u_BC) as the next step's f(0,y,z) and its
-//     own post-collision populations into `inlet_raw`.
-//   * ADDirichlet inlet (post-stream): the x = 0 thread stores phi_in * w.
-//   * Pre-collision inlet series (ADDirichletTanhPulse pattern): every pull whose
-//     source cell (or keep-local cell) lies on x = 0 uses phi_in(step) * w_q.
-//   * The LDF of step s reads phi written by step s-1's AD collision; phi is kept
-//     in place as state (read, then overwritten with this step's value).
-//   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet threads add their phi
-//     into FMX_OUTLET_SLOTS FP64 accumulators per step.
+//     feq(rho_N, u_BC) + f_N - feq(rho_N, u_N), f_N the populations of (1,y,z) after step s
+//     (VelocityDirichletNEQ.cu:49-96).  The x = 0 lane takes f_N and its moments from the x = 1
+//     lane at the end of step s and stores the result as the next step's f(0,y,z); on the last
+//     step of a run it also stores its own post-collision populations into the `inlet_raw` plane,
+//     which is what the reference holds in f(0,y,z) between steps (state read-back).
+//     fmx_coupled_prologue applies the first step's update.
+//   * Equilibrium velocity inlet (VelocityDirichletEq, VelocityDirichlet.cu:54-84): the x = 0
+//     lane stores feq(rho0, u_BC) as the next step's f(0,y,z) (and `inlet_raw` as above).
+//   * ADDirichlet inlet (post-stream): the x = 0 lane stores phi_in * w.
+//   * Pre-collision inlet series (ADDirichletTanhPulse pattern): every pull whose source cell
+//     (or keep-local cell) lies on x = 0 uses phi_in(step) * w_q.
+//   * The LDF of step s reads phi written by step s-1's AD collision (phi ping-pong).
+//   * Outlet breakthrough (OutletCSV.hpp:51-80): the outlet lanes add their phi into
+//     FMX_OUTLET_SLOTS FP64 accumulators per step.
+// When x = nx-2 and nx-1 fall in different waves ((nx - 1) % 64 == 0) the x = 1 and x = nx-2
+// threads write the neighbouring boundary cell's populations instead.
 #include "coupled_kernel.hpp"
 
 namespace fmx {
@@ -272,7 +270,8 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
     using C = typename Storage<P>::C;
     const CoupledParams<C> prm = make_params<C>(d);
     for (int k = 0; k < nsteps; ++k) {
-        const CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
+        CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
+        if (k != nsteps - 1) ptr.inlet_raw = nullptr;  // the raw x = 0 plane is state read-back only
         if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, acc_row0 + k, s)) return rc;
     }
     return FMX_OK;
@@ -282,6 +281,10 @@ template int run_coupled_steps<FMX_F32>(const fmx_coupled_desc &, const fmx_coup
                                         i64);
 template int run_coupled_steps<FMX_F64>(const fmx_coupled_desc &, const fmx_coupled_bufs &, i64, int, int, hipStream_t,
                                         i64);
+template int run_coupled_steps<FMX_F16>(const fmx_coupled_desc &, const fmx_coupled_bufs &, i64, int, int, hipStream_t,
+                                        i64);
+template int run_coupled_steps<FMX_BF16>(const fmx_coupled_desc &, const fmx_coupled_bufs &, i64, int, int, hipStream_t,
+                                         i64);
 
 template <int P>
 static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s) {

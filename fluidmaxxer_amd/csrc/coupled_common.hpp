@@ -18,6 +18,7 @@ template <class C> struct CoupledParams {
     int xcd_remap;
     int tile_y, tile_z;      // row tiles (0: plain y-then-z order); they divide gy and gz
     int pair;                // two cells per thread (k_coupled_pair) where the storage and nx allow it
+    int lane_bc;             // boundary lanes take their x +- 1 neighbour's values by lane shuffles
     i64 n;
     C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
 };
@@ -92,9 +93,17 @@ __device__ __forceinline__ typename Storage<P>::C decode(typename Storage<P>::S 
 template <int P>
 __device__ __forceinline__ typename Storage<P>::S encode(typename Storage<P>::C v, typename Storage<P>::C inv_w,
                                                           typename Storage<P>::C ref) {
+    using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
-    if constexpr (IsHalf<P>::value) return S(v * inv_w - ref);
-    else return v;
+    if constexpr (IsHalf<P>::value) {
+        C t = v * inv_w - ref;
+        // keep the 16-bit rounding a separate conversion: fused into a mixed-precision FMA it writes
+        // half a register, which serialises on the register's previous value
+        asm volatile("" : "+v"(t));
+        return S(t);
+    } else {
+        return v;
+    }
 }
 
 // feq in the BGK form w rho (1 + cu + 0.5 cu^2 - 1.5 u^2), cu = 3 e.u (BGKCollision.cu:166-173)
@@ -165,6 +174,9 @@ template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
     const char *pe = std::getenv("FMX_PAIR");
     const char *fz = std::getenv("FMX_FREEZE");
     p.pair = (pe && std::atoi(pe) != 0) && !(fz && std::atoi(fz) != 0) && d.nx >= 128 && d.nx % 128 == 0;
+    // the x = nx-2 / nx-1 and x = 0 / 1 cells share a wave when rows start at wave boundaries and
+    // nx - 1 is not a multiple of 64
+    p.lane_bc = (ls.bx % 64 == 0) && ((d.nx - 1) % 64 != 0);
     p.n = i64(d.nx) * d.ny * d.nz;
     C tau = C(d.tau);
     if (tau < C(0.5001)) tau = C(0.5001);  // BGKCollision.cu:203-204

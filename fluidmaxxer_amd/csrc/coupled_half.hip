@@ -5,12 +5,17 @@
 // ALU issue rather than memory, and these halve the arithmetic's instruction count while changing
 // results far below the storage rounding (measured against the FP64 restatement).
 #include "coupled_kernel.hpp"
+#include "coupled_pair.hpp"
 
 namespace fmx {
 
 template int launch_coupled_step<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
                                           hipStream_t);
 template int launch_coupled_step<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
+                                           hipStream_t);
+template int launch_coupled_pair<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
+                                          hipStream_t);
+template int launch_coupled_pair<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
                                            hipStream_t);
 
 }  // namespace fmx

@@ -119,36 +119,67 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     // ------------------------------------------------------------- NS stores
     const bool inlet_cell = prm.ns_inlet && x == 0;
     const bool outlet_cell = prm.ns_outlet && x == nx - 1;
-    if (inlet_cell) {
+    if (inlet_cell && p.inlet_raw) {  // the x = 0 populations as the reference holds them (last step of a run)
         const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
-        if (p.inlet_raw) {
 #pragma unroll
-            for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+        for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+    }
+    if (prm.lane_bc) {
+        // Boundary values computed by the boundary lane itself, from its x +- 1 neighbour's values
+        // handed over by lane shuffles (the neighbour is the adjacent lane of the same wave), so every
+        // population plane takes one full-wave store:
+        //   x = 0:      next step's inlet f: feq(rho0, u_BC), or NEQ from the x = 1 populations
+        //   x = nx-1:   PressureDirichlet feq(rho_out, u) with u of the x = nx-2 cell
+        C um_x = C(0), um_y = C(0), um_z = C(0);
+        if (prm.ns_outlet) {
+            um_x = __shfl_up(ux2, 1, 64);
+            um_y = __shfl_up(uy2, 1, 64);
+            um_z = __shfl_up(uz2, 1, 64);
         }
-        if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
+        C fp[19], rp = C(0), up_x = C(0), up_y = C(0), up_z = C(0);
+        if (prm.ns_inlet == 1) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) fp[q] = __shfl_down(fo[q], 1, 64);
+            rp = __shfl_down(r2, 1, 64);
+            up_x = __shfl_down(ux2, 1, 64);
+            up_y = __shfl_down(uy2, 1, 64);
+            up_z = __shfl_down(uz2, 1, 64);
+        }
+        if (!frozen) static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C v = fo[q];
+            if (inlet_cell) v = (prm.ns_inlet == 2) ? inlet_eq<q>(prm) : inlet_neq<q>(fp[q], rp, up_x, up_y, up_z, prm);
+            else if (outlet_cell) v = outlet_pressure<q>(um_x, um_y, um_z, prm);
+            st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+        });
+    } else {
+        // the x = 1 thread writes the NEQ inlet of x = 0, the x = nx-2 thread the pressure outlet
+        if (inlet_cell) {
+            if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
+                });
+            }
+        } else if (!outlet_cell && !frozen) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q)
+                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
+        }
+        if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
+                st_b(plane(p.fn, q, n), cell_b - SB,
+                     encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
             });
         }
-    } else if (!outlet_cell && !frozen) {
-#pragma unroll
-        for (int q = 0; q < 19; ++q)
-            st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
-    }
-    if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            st_b(plane(p.fn, q, n), cell_b - SB,
-                 encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
-        });
-    }
-    if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            st_b(plane(p.fn, q, n), cell_b + SB,
-                 encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
-        });
+        if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                st_b(plane(p.fn, q, n), cell_b + SB,
+                     encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
+            });
+        }
     }
 
     // ------------------------------------------------------ sorption source and AD collision
@@ -165,19 +196,42 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     st_b(p.phi_new, cell * CB, phi);
 
     // ------------------------------------------------------------- AD stores
-    // (half storage: deviation about this cell's new concentration, which the readers load as phi)
+    // (half storage: deviation about this cell's new concentration, which the readers load as phi; the
+    // Neumann outlet cell holds its x - 1 neighbour's populations, about that neighbour's concentration)
+    const bool ad_in_cell = prm.ad_inlet == 1 && x == 0;
     const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
-    if (prm.ad_inlet == 1 && x == 0) {
+    if (prm.lane_bc) {
+        C gm[7], phim = C(0);
+        if (prm.ad_outlet) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q)
-            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
-    } else if (!ad_out_cell) {
+            for (int q = 0; q < 7; ++q) gm[q] = __shfl_up(go[q], 1, 64);
+            phim = __shfl_up(phi, 1, 64);
+        }
+        static_for<0, 7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C v = go[q], ref = phi;
+            if (ad_in_cell) {
+                v = prm.phi_in * w7<C>(q);
+            } else if (ad_out_cell) {
+                v = gm[q];
+                ref = phim;
+            }
+            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(v, C(Q7::inv_w[q]), ref));
+        });
+    } else {
+        if (ad_in_cell) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
-    }
-    if (prm.ad_outlet && x == nx - 2) {
+            for (int q = 0; q < 7; ++q)
+                st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
+        } else if (!ad_out_cell) {
 #pragma unroll
-        for (int q = 0; q < 7; ++q) st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+            for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+        }
+        if (prm.ad_outlet && x == nx - 2) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+        }
     }
 
     // ------------------------------------------------- outlet breakthrough sum

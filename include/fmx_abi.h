@@ -222,7 +222,8 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
 
 /* ------------------------------------------------------ discrete adjoint */
 /* No counterpart in the reference (it has no adjoint): the exact reverse-mode derivative of the
- * fused step above (fmx_coupled_run), for FP32 or FP64 state, equilibrium or no NS inlet.
+ * fused step above (fmx_coupled_run), equilibrium or no NS inlet.  With FP16 / BF16 storage the
+ * checkpoints and segment states keep the 16-bit deviation encoding and the reverse steps run in FP32.
  *
  * Loss: any L whose dependence on the run is through the outlet means C_out(s) (fmx_outlet_mean);
  * the caller supplies outlet_weight[s - step0] = dL/dC_out(s).  Gradients returned:
@@ -247,7 +248,7 @@ typedef struct fmx_adjoint_bufs {
     double *grad_series;         /* indexed like inlet_series; entries step0 .. step0+nsteps-1 written, or NULL */
 } fmx_adjoint_bufs;
 
-/* Bytes of one stored state (f, g, q, phi; each section 256-byte aligned); -1 for half storage */
+/* Bytes of one stored state (f, g, q, phi in the storage precision; each section 256-byte aligned) */
 int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *desc);
 /* fmx_coupled_run with checkpoints: copies the state before steps step0 + j*k into slot j of ckpt,
  * outlet sums as fmx_coupled_run (row s - step0); result in buffer (cur + nsteps) % 2.  Clears the

@@ -325,11 +325,17 @@ def test_c4_differentiation_fp64_vs_fd():
     # bound is 1e-5 (observed 2e-6)
     assert rel(ad, fd) < 1e-5
     # fluid cells sit at field = 0, where the gradient passed through is the one-sided (upward) one
+    # (forward differences at h and 2h, extrapolated: 2 D(h) - D(2h) cancels the O(h) term, which the
+    # drag of partial bounce-back in every fluid cell makes large)
     w = np.abs(np.random.default_rng(8).standard_normal(cfg.n)) * (f0 == 0)
     h = 1e-7
-    fd1 = (pr.loss(field=f0 + h * w) - pr.loss()) / h
+    l0 = pr.loss()
+    d1 = (pr.loss(field=f0 + h * w) - l0) / h
+    d2 = (pr.loss(field=f0 + 2 * h * w) - l0) / (2 * h)
+    fd1 = 2 * d1 - d2
     ad1 = float(g["field"] @ w)
-    say(f"    <dL/dfield, w> (field = 0, upward): adjoint {ad1:.10e} one-sided fd {fd1:.10e} rel {rel(ad1, fd1):.2e}")
+    say(f"    <dL/dfield, w> (field = 0, upward): adjoint {ad1:.10e} one-sided fd {fd1:.10e} (D(h) {d1:.6e}) "
+        f"rel {rel(ad1, fd1):.2e}")
     assert rel(ad1, fd1) < 1e-4
 
 
@@ -350,3 +356,60 @@ def test_c4_fp32_adjoint_vs_fp64_adjoint():
     e = np.linalg.norm(g32["field"] - g64["field"]) / np.linalg.norm(g64["field"])
     say(f"  C4 f32 vs f64 field gradient: relative l2 {e:.2e}")
     assert e < 1e-2
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+@pytest.mark.parametrize("prec,bound_iso,bound_vec", [("f16", 1e-2, 5e-2), ("bf16", 5e-2, 2e-1)])
+def test_half_state_adjoint_against_fp64(pbb_problem, prec, bound_iso, bound_vec):
+    """16-bit deviation-stored checkpoints and segment states (FP32 arithmetic in the recompute and
+    the adjoint) against the FP64 adjoint; the errors are reported."""
+    g64 = pbb_problem.grads()
+    ph = Problem(adj_config("pbb"), prec=prec, target=pbb_problem.target)
+    gh = ph.grads()
+    errs = {name: rel(gh[name], g64[name]) for name in ("iso_a", "iso_b")}
+    errs.update({name: rel_l2(gh[name], g64[name]) for name in ("field", "series")})
+    say(f"  {prec} states vs f64 adjoint:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["iso_a"] < bound_iso and errs["iso_b"] < bound_iso, errs
+    assert errs["field"] < bound_vec and errs["series"] < bound_vec, errs
+
+
+def interpolation_matrix(n_knots, steps):
+    """d(per-step inlet value)/d(knot) of workloads.series_per_step (linear interpolation)."""
+    dt = steps / (n_knots - 1)
+    s = np.arange(steps + 1) / dt
+    B = np.zeros((steps + 1, n_knots))
+    i0 = np.minimum(np.floor(s).astype(int), n_knots - 2)
+    t = s - i0
+    B[np.arange(steps + 1), i0] = 1 - t
+    B[np.arange(steps + 1), i0 + 1] = t
+    return B
+
+
+def test_c5_half_checkpoints_against_fp64():
+    """C5 physics (overlapping sphere pack, partial bounce-back, Langmuir, 200-knot inlet series) on a
+    128x64x64 box for 4000 steps: the FP16-stored adjoint with checkpoints every 500 steps (FP32
+    recompute) against the FP64 adjoint (checkpoints every 200); gradients with respect to the 200
+    knots (chain rule through the per-step interpolation) and the per-cell solid fraction.  As for
+    C4, the fluid cells' solid fraction is set to 0 and the concentration clamp to 0 so that the
+    loss has something to differentiate (see test_c4_differentiation_fp64_vs_fd)."""
+    steps = 4000
+    cfg = workloads.build("C5", dims=(128, 64, 64), steps=steps)
+    cfg.field = np.where(cfg.field > 0.5, cfg.field, 0.0)
+    cfg.min_conc = 0.0
+    knots = cfg.meta["knots"]
+    B = interpolation_matrix(len(knots), steps)
+    assert np.allclose(B @ knots, cfg.inlet_series)
+    p64 = Problem(cfg, prec="f64", nsteps=steps, k=200)
+    p64.target = 0.85 * p64.c0
+    g64 = p64.grads()
+    p16 = Problem(cfg, prec="f16", nsteps=steps, k=500, target=p64.target)
+    g16 = p16.grads()
+    k64, k16 = B.T @ g64["series"], B.T @ g16["series"]
+    errs = dict(knots=rel_l2(k16, k64), field=rel_l2(g16["field"], g64["field"]),
+                c_out=rel_l2(p16.c0, p64.c0))
+    say(f"  C5 128x64x64 f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
+        f"|dL/dknots| {np.linalg.norm(k64):.3e}")
+    assert errs["knots"] < 5e-2 and errs["field"] < 1e-1, errs
