@@ -389,27 +389,29 @@ def interpolation_matrix(n_knots, steps):
 
 
 def test_c5_half_checkpoints_against_fp64():
-    """C5 physics (overlapping sphere pack, partial bounce-back, Langmuir, 200-knot inlet series) on a
-    128x64x64 box for 4000 steps: the FP16-stored adjoint with checkpoints every 500 steps (FP32
-    recompute) against the FP64 adjoint (checkpoints every 200); gradients with respect to the 200
-    knots (chain rule through the per-step interpolation) and the per-cell solid fraction.  As for
-    C4, the fluid cells' solid fraction is set to 0 and the concentration clamp to 0 so that the
-    loss has something to differentiate (see test_c4_differentiation_fp64_vs_fd)."""
-    steps = 4000
-    cfg = workloads.build("C5", dims=(128, 64, 64), steps=steps)
+    """C5 physics (overlapping sphere pack at porosity 0.4, partial bounce-back, Langmuir, 200-knot inlet
+    series) on a 96x64x64 box for 8000 steps: the FP16-stored adjoint with checkpoints every 500 steps
+    (FP32 recompute) against the FP64 adjoint (checkpoints every 200); gradients with respect to the 200
+    knots (chain rule through the per-step interpolation) and the per-cell solid fraction.  As for C4,
+    the fluid cells' solid fraction and the concentration clamp are set to 0 so that the loss has a
+    breakthrough curve to differentiate (see c4_config); the box is shortened so that the front reaches
+    the outlet within the run."""
+    steps = 8000
+    cfg = workloads.build("C5", dims=(96, 64, 64), steps=steps)
     cfg.field = np.where(cfg.field > 0.5, cfg.field, 0.0)
     cfg.min_conc = 0.0
     knots = cfg.meta["knots"]
     B = interpolation_matrix(len(knots), steps)
     assert np.allclose(B @ knots, cfg.inlet_series)
     p64 = Problem(cfg, prec="f64", nsteps=steps, k=200)
+    say(f"  C5 96x64x64 c_out at 2000/4000/6000/8000: {[float('%.3e' % p64.c0[s - 1]) for s in (2000, 4000, 6000, 8000)]}")
+    assert p64.c0[-1] > 1e-3
     p64.target = 0.85 * p64.c0
     g64 = p64.grads()
     p16 = Problem(cfg, prec="f16", nsteps=steps, k=500, target=p64.target)
     g16 = p16.grads()
     k64, k16 = B.T @ g64["series"], B.T @ g16["series"]
-    errs = dict(knots=rel_l2(k16, k64), field=rel_l2(g16["field"], g64["field"]),
-                c_out=rel_l2(p16.c0, p64.c0))
-    say(f"  C5 128x64x64 f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
+    errs = dict(knots=rel_l2(k16, k64), field=rel_l2(g16["field"], g64["field"]), c_out=rel_l2(p16.c0, p64.c0))
+    say(f"  C5 f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
         f"|dL/dknots| {np.linalg.norm(k64):.3e}")
     assert errs["knots"] < 5e-2 and errs["field"] < 1e-1, errs
