@@ -3,7 +3,9 @@
 `AdjointRun` owns one coupled configuration on the GPU, a checkpoint store and the
 adjoint buffers.  `forward()` runs the configuration from its initial state for
 `nsteps` steps (fmx_adjoint_forward, a checkpoint every `k` steps) and returns the
-outlet-averaged concentration C_out(s), s = 1..nsteps.  `backward(w)` runs the
+outlet-averaged concentration C_out(s), s = 1..nsteps.  Checkpoints keep the storage
+precision, or, for an FP32 run, `ckpt="f16"` / `"bf16"` stores them as 16-bit
+deviations and the reverse sweep recomputes each segment in FP32 from them.  `backward(w)` runs the
 reverse sweep (fmx_adjoint_backward) for a loss with dL/dC_out(s) = w[s-1] and
 returns the gradients with respect to the isotherm constants (Langmuir q_max and
 K_L, or the linear K_Henry), the per-cell field (partial-bounce-back solid fraction
@@ -32,10 +34,14 @@ def _torch():
 
 
 class AdjointRun:
-    def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0"):
+    def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0",
+                 ckpt: str = None):
         torch = _torch()
         if prec not in ("f32", "f64", "f16", "bf16"):
             raise abi.FmxError(f"unknown precision {prec}")
+        ckpt = prec if ckpt is None else ckpt
+        if ckpt != prec and not (prec == "f32" and ckpt in ("f16", "bf16")):
+            raise abi.FmxError(f"checkpoints of a {prec} run are stored in {prec} (or f16 / bf16 for f32)")
         if cfg.ns_inlet == "neq":
             raise abi.FmxError("the NEQ velocity inlet has no adjoint; use ns_inlet='eq'")
         if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):
@@ -47,10 +53,13 @@ class AdjointRun:
         n = cfg.n
         d = self.sim._desc
         self.state_bytes = int(abi.lib().fmx_adjoint_state_bytes(abi.C.byref(d)))
+        self.ckpt_prec = abi.PREC_NAMES[ckpt]
+        self.ckpt_bytes = int(abi.lib().fmx_adjoint_ckpt_bytes(abi.C.byref(d), self.ckpt_prec))
         nseg = (self.nsteps + self.k - 1) // self.k
+        nslots = self.k if ckpt != prec else max(self.k - 1, 1)
         u8 = torch.uint8
-        self.ckpt = torch.empty(nseg * self.state_bytes, dtype=u8, device=self.device)
-        self.seg = torch.empty(max(self.k - 1, 1) * self.state_bytes, dtype=u8, device=self.device)
+        self.ckpt = torch.empty(nseg * self.ckpt_bytes, dtype=u8, device=self.device)
+        self.seg = torch.empty(nslots * self.state_bytes, dtype=u8, device=self.device)
         self.lam_f = [torch.empty(19 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
         self.lam_g = [torch.empty(7 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
         self.lam_q = torch.empty(n, dtype=self.cdt, device=self.device)
@@ -91,7 +100,7 @@ class AdjointRun:
         abi.call("fmx_coupled_init", sim._desc_p(), sim._bufs_p(), 0, float(self.cfg.phi0), sim._s())
         self.acc = sim.outlet_acc(self.nsteps)
         abi.call("fmx_adjoint_forward", sim._desc_p(), sim._bufs_p(self.acc), 1, self.nsteps, self.k, 0,
-                 self.ckpt.data_ptr(), sim._s())
+                 self.ckpt.data_ptr(), self.ckpt_prec, sim._s())
         sim.cur = self.nsteps & 1
         sim.step_index = self.nsteps
         mean = torch.empty(self.nsteps, dtype=torch.float64, device=self.device)
@@ -120,7 +129,7 @@ class AdjointRun:
         a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
         sim = self.sim
         abi.call("fmx_adjoint_backward", sim._desc_p(), sim._bufs_p(), 1, self.nsteps, self.k, self.ckpt.data_ptr(),
-                 self.seg.data_ptr(), abi.C.byref(a), sim._s())
+                 self.ckpt_prec, self.seg.data_ptr(), abi.C.byref(a), sim._s())
         sim.synchronize()
         gi = self.grad_iso.cpu().numpy()
         return {

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                     // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
                     // on a Neumann outlet plane), as the forward step stores them
                     const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                    gi[q] = w7<C>(q) * (p.phi[sref] + C(p.g[i64(q) * n + s]));
+                    gi[q] = w7<C>(q) * (p.phi[sref] + widen(p.g[i64(q) * n + s]));
                 } else {
                     gi[q] = p.g[i64(q) * n + s];
                 }
@@ -441,6 +441,42 @@ __global__ void __launch_bounds__(64) k_iso_final(const double *__restrict__ acc
     }
 }
 
+// 16-bit checkpoints of an FP32 run: f about f_ref and g about the source cell's concentration
+// (this cell's, or the x - 1 cell's on a Neumann outlet plane, whose populations it holds), the
+// layout the 16-bit forward step itself stores; q and phi are copied in FP32.
+template <int CP>
+__global__ void __launch_bounds__(256) k_ckpt_encode(const float *__restrict__ f, const float *__restrict__ g,
+                                                     const float *__restrict__ phi,
+                                                     typename Storage<CP>::S *__restrict__ hf,
+                                                     typename Storage<CP>::S *__restrict__ hg, i64 n, int nx,
+                                                     int ad_outlet, float f_ref) {
+    using S = typename Storage<CP>::S;
+    const i64 c = gtid();
+    if (c >= n) return;
+    const i64 sref = (ad_outlet && c % nx == nx - 1) ? c - 1 : c;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) hf[i64(q) * n + c] = S(f[i64(q) * n + c] * float(Q19::inv_w[q]) - f_ref);
+    const float ph = phi[sref];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) hg[i64(q) * n + c] = S(g[i64(q) * n + c] * float(Q7::inv_w[q]) - ph);
+}
+
+template <int CP>
+__global__ void __launch_bounds__(256) k_ckpt_decode(const typename Storage<CP>::S *__restrict__ hf,
+                                                     const typename Storage<CP>::S *__restrict__ hg,
+                                                     const float *__restrict__ phi, float *__restrict__ f,
+                                                     float *__restrict__ g, i64 n, int nx, int ad_outlet,
+                                                     float f_ref) {
+    const i64 c = gtid();
+    if (c >= n) return;
+    const i64 sref = (ad_outlet && c % nx == nx - 1) ? c - 1 : c;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[i64(q) * n + c] = w19<float>(q) * (f_ref + widen(hf[i64(q) * n + c]));
+    const float ph = phi[sref];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) g[i64(q) * n + c] = w7<float>(q) * (ph + widen(hg[i64(q) * n + c]));
+}
+
 // ------------------------------------------------------------- host side
 inline i64 align256(i64 b) { return (b + 255) & ~i64(255); }
 
@@ -484,6 +520,41 @@ template <int P> static int save_state(const fmx_coupled_bufs &b, int cur, const
         return rc;
     if (int rc = check_hip(hipMemcpyAsync(dst.q, b.q, n * sizeof(C), hipMemcpyDeviceToDevice, s), "save q")) return rc;
     return check_hip(hipMemcpyAsync(dst.phi, b.phi[cur], n * sizeof(C), hipMemcpyDeviceToDevice, s), "save phi");
+}
+
+// The current FP32 state of b into a 16-bit checkpoint slot.
+template <int CP>
+static int save_half_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, const StateView<CP> &dst,
+                          i64 n, hipStream_t s) {
+    hipLaunchKernelGGL(k_ckpt_encode<CP>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const float *>(b.f[cur]),
+                       static_cast<const float *>(b.g[cur]), static_cast<const float *>(b.phi[cur]), dst.f, dst.g, n,
+                       d.nx, d.ad_outlet, float(d.f_ref));
+    if (int rc = after_launch("k_ckpt_encode")) return rc;
+    if (int rc = check_hip(hipMemcpyAsync(dst.q, b.q, n * sizeof(float), hipMemcpyDeviceToDevice, s), "save q")) return rc;
+    return check_hip(hipMemcpyAsync(dst.phi, b.phi[cur], n * sizeof(float), hipMemcpyDeviceToDevice, s), "save phi");
+}
+
+// A 16-bit checkpoint slot decoded into an FP32 state slot.
+template <int CP>
+static int load_half_ckpt(const fmx_coupled_desc &d, const StateView<CP> &src, const StateView<FMX_F32> &dst, i64 n,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_ckpt_decode<CP>, dim3(blocks_for(n)), dim3(256), 0, s, src.f, src.g, src.phi, dst.f, dst.g,
+                       n, d.nx, d.ad_outlet, float(d.f_ref));
+    if (int rc = after_launch("k_ckpt_decode")) return rc;
+    if (int rc = check_hip(hipMemcpyAsync(dst.q, src.q, n * sizeof(float), hipMemcpyDeviceToDevice, s), "load q"))
+        return rc;
+    return check_hip(hipMemcpyAsync(dst.phi, src.phi, n * sizeof(float), hipMemcpyDeviceToDevice, s), "load phi");
+}
+
+// The state of b into checkpoint slot j: as stored, or (FP32 run, 16-bit checkpoints) encoded.
+template <int P>
+static int save_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, void *ckpt, int cp, i64 j, i64 n,
+                     hipStream_t s) {
+    if constexpr (P == FMX_F32) {
+        if (cp == FMX_F16) return save_half_ckpt<FMX_F16>(d, b, cur, state_at<FMX_F16>(ckpt, j, n), n, s);
+        if (cp == FMX_BF16) return save_half_ckpt<FMX_BF16>(d, b, cur, state_at<FMX_BF16>(ckpt, j, n), n, s);
+    }
+    return save_state<P>(b, cur, state_at<P>(ckpt, j, n), n, s);
 }
 
 // One forward step from stored state a into stored state b (no outlet sums, no inlet_raw).
@@ -550,9 +621,17 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
     return after_launch("k_adjoint");
 }
 
+// Checkpoints at the storage precision, or 16-bit checkpoints of an FP32 run.
+static bool ckpt_prec_ok(int prec, int cp) { return cp == prec || (prec == FMX_F32 && (cp == FMX_F16 || cp == FMX_BF16)); }
+
 static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
-                         int32_t k, const char *who) {
+                         int32_t k, int cp, const char *who) {
     if (int rc = fmx_coupled_validate(d, b)) return rc;
+    if (!ckpt_prec_ok(d->prec, cp))
+        return set_error(FMX_ERR_UNSUPPORTED,
+                         "%s: checkpoint precision %d with storage precision %d (checkpoints keep the storage "
+                         "precision, or FP16 / BF16 for an FP32 run)",
+                         who, cp, d->prec);
 
     if (d->ns_inlet == 1)
         return set_error(FMX_ERR_UNSUPPORTED, "%s: the NEQ velocity inlet has no adjoint (use the equilibrium inlet)",
@@ -563,13 +642,13 @@ static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
 
 template <int P>
 static int forward_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int k, int cur,
-                        void *ckpt, hipStream_t s) {
+                        void *ckpt, int cp, hipStream_t s) {
     const i64 n = i64(d.nx) * d.ny * d.nz;
     if (int rc = clear_frozen(d, b.links, s)) return rc;
     const int nseg = (nsteps + k - 1) / k;
     for (int j = 0; j < nseg; ++j) {
         const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
-        if (int rc = save_state<P>(b, cur, state_at<P>(ckpt, j, n), n, s)) return rc;
+        if (int rc = save_ckpt<P>(d, b, cur, ckpt, cp, j, n, s)) return rc;
         if (int rc = run_coupled_steps<P>(d, b, step0 + i64(j) * k, len, cur, s, i64(j) * k)) return rc;
         cur = (cur + len) & 1;
     }
@@ -578,7 +657,7 @@ static int forward_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i6
 
 template <int P>
 static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int k,
-                    void *ckpt, void *seg, const fmx_adjoint_bufs &adj, hipStream_t s) {
+                    void *ckpt, int cp, void *seg, const fmx_adjoint_bufs &adj, hipStream_t s) {
     using C = typename Storage<P>::C;
     const i64 n = i64(d.nx) * d.ny * d.nz;
     const CoupledParams<C> prm = make_params<C>(d);
@@ -602,8 +681,21 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
     for (int j = nseg - 1; j >= 0; --j) {
         const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
         const i64 s0 = step0 + i64(j) * k;
-        // states at the start of steps s0 .. s0+len-1: the checkpoint, then seg slots 0 .. len-2
-        auto state = [&](int i) { return i == 0 ? state_at<P>(ckpt, j, n) : state_at<P>(seg, i - 1, n); };
+        // states at the start of steps s0 .. s0+len-1: the checkpoint, then seg slots 0 .. len-2; a
+        // 16-bit checkpoint of an FP32 run is decoded into seg slot 0 and the states follow in 1 .. len-1
+        const bool recast = cp != P;
+        if constexpr (P == FMX_F32) {
+            if (cp == FMX_F16)
+                if (int rc = load_half_ckpt<FMX_F16>(d, state_at<FMX_F16>(ckpt, j, n), state_at<P>(seg, 0, n), n, s))
+                    return rc;
+            if (cp == FMX_BF16)
+                if (int rc = load_half_ckpt<FMX_BF16>(d, state_at<FMX_BF16>(ckpt, j, n), state_at<P>(seg, 0, n), n, s))
+                    return rc;
+        }
+        auto state = [&](int i) {
+            if (recast) return state_at<P>(seg, i, n);
+            return i == 0 ? state_at<P>(ckpt, j, n) : state_at<P>(seg, i - 1, n);
+        };
         for (int i = 1; i < len; ++i)
             if (int rc = step_between<P>(d, b, state(i - 1), state(i), prm, s0 + i - 1, s)) return rc;
         for (int i = len - 1; i >= 0; --i) {
@@ -633,26 +725,38 @@ int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *d) {
     }
 }
 
+int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *d, int ckpt_prec) {
+    if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1 || !ckpt_prec_ok(d->prec, ckpt_prec)) return -1;
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    switch (ckpt_prec) {
+        case FMX_F32: return state_bytes<FMX_F32>(n);
+        case FMX_F64: return state_bytes<FMX_F64>(n);
+        case FMX_F16: return state_bytes<FMX_F16>(n);
+        default: return state_bytes<FMX_BF16>(n);
+    }
+}
+
 int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
-                        int32_t k, int cur, void *ckpt, void *stream) {
+                        int32_t k, int cur, void *ckpt, int ckpt_prec, void *stream) {
     FMX_TRY_BEGIN
-    if (int rc = adjoint_check(d, b, step0, nsteps, k, "fmx_adjoint_forward")) return rc;
+    if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_forward")) return rc;
     if ((cur != 0 && cur != 1) || !ckpt) return set_error(FMX_ERR_ARG, "fmx_adjoint_forward: bad cur or checkpoint store");
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
-        case FMX_F32: return forward_ckpt<FMX_F32>(*d, *b, step0, nsteps, k, cur, ckpt, s);
-        case FMX_F64: return forward_ckpt<FMX_F64>(*d, *b, step0, nsteps, k, cur, ckpt, s);
-        case FMX_F16: return forward_ckpt<FMX_F16>(*d, *b, step0, nsteps, k, cur, ckpt, s);
-        default: return forward_ckpt<FMX_BF16>(*d, *b, step0, nsteps, k, cur, ckpt, s);
+        case FMX_F32: return forward_ckpt<FMX_F32>(*d, *b, step0, nsteps, k, cur, ckpt, ckpt_prec, s);
+        case FMX_F64: return forward_ckpt<FMX_F64>(*d, *b, step0, nsteps, k, cur, ckpt, ckpt_prec, s);
+        case FMX_F16: return forward_ckpt<FMX_F16>(*d, *b, step0, nsteps, k, cur, ckpt, ckpt_prec, s);
+        default: return forward_ckpt<FMX_BF16>(*d, *b, step0, nsteps, k, cur, ckpt, ckpt_prec, s);
     }
     FMX_TRY_END
 }
 
 int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
-                         int32_t k, const void *ckpt, void *seg, const fmx_adjoint_bufs *adj, void *stream) {
+                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, const fmx_adjoint_bufs *adj,
+                         void *stream) {
     FMX_TRY_BEGIN
-    if (int rc = adjoint_check(d, b, step0, nsteps, k, "fmx_adjoint_backward")) return rc;
-    if (!ckpt || (k > 1 && nsteps > 1 && !seg) || !adj)
+    if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_backward")) return rc;
+    if (!ckpt || (((k > 1 && nsteps > 1) || ckpt_prec != d->prec) && !seg) || !adj)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing checkpoint or segment store");
     if (!adj->lam_f[0] || !adj->lam_f[1] || !adj->lam_g[0] || !adj->lam_g[1] || !adj->lam_q || !adj->lam_phi ||
         !adj->iso_acc || !adj->grad_iso)
@@ -662,10 +766,10 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
     switch (d->prec) {
-        case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
-        case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
-        case FMX_F16: return backward<FMX_F16>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
-        default: return backward<FMX_BF16>(*d, *b, step0, nsteps, k, ck, seg, *adj, s);
+        case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
+        case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
+        case FMX_F16: return backward<FMX_F16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
+        default: return backward<FMX_BF16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
     }
     FMX_TRY_END
 }

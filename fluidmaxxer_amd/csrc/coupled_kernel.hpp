@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
                 // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
                 // populations of its x - 1 neighbour, stored about that neighbour's concentration
                 const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                gi[q] = w7<C>(q) * (ld_b(p.phi, sref * CB) + C(raw));
+                gi[q] = w7<C>(q) * (ld_b(p.phi, sref * CB) + widen(raw));
             } else {
                 gi[q] = raw;
             }

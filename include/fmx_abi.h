@@ -232,8 +232,12 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  *                   on [0, 1])
  *   grad_series[s]  dL/d inlet_series[s] (ad_inlet = 2)
  * Checkpointing: fmx_adjoint_forward keeps the state before steps step0, step0+k, ... in `ckpt`
- * (ceil(nsteps/k) slots of fmx_adjoint_state_bytes); fmx_adjoint_backward recomputes each segment
- * into `seg` (k-1 slots) and runs the adjoint steps over it in reverse. */
+ * (ceil(nsteps/k) slots of fmx_adjoint_ckpt_bytes); fmx_adjoint_backward recomputes each segment
+ * into `seg` (slots of fmx_adjoint_state_bytes) and runs the adjoint steps over it in reverse.
+ * Checkpoint precision `ckpt_prec`: the storage precision (desc->prec: states kept as they are), or,
+ * for an FP32 run, FMX_F16 / FMX_BF16: f and g are checkpointed as 16-bit deviations (f/w - f_ref,
+ * g/w - phi_src, the 16-bit storage layout of the forward step), q and phi in FP32, and each segment
+ * is recomputed in FP32 from its decoded checkpoint (seg then holds k slots instead of k-1). */
 #define FMX_ADJ_SLOTS 64
 #define FMX_ADJ_STRIDE 16 /* doubles between accumulator slots */
 
@@ -250,16 +254,21 @@ typedef struct fmx_adjoint_bufs {
 
 /* Bytes of one stored state (f, g, q, phi in the storage precision; each section 256-byte aligned) */
 int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *desc);
-/* fmx_coupled_run with checkpoints: copies the state before steps step0 + j*k into slot j of ckpt,
- * outlet sums as fmx_coupled_run (row s - step0); result in buffer (cur + nsteps) % 2.  Clears the
- * frozen-solid link bits (the adjoint needs every cell on the full update path). */
+/* Bytes of one checkpoint slot at precision ckpt_prec; -1 if the run's storage cannot be
+ * checkpointed at that precision */
+int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *desc, int ckpt_prec);
+/* fmx_coupled_run with checkpoints: stores the state before steps step0 + j*k into slot j of ckpt
+ * (at ckpt_prec), outlet sums as fmx_coupled_run (row s - step0); result in buffer
+ * (cur + nsteps) % 2.  Clears the frozen-solid link bits (the adjoint needs every cell on the full
+ * update path). */
 int fmx_adjoint_forward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
-                        int32_t k, int cur, void *ckpt, void *stream);
-/* Reverse sweep over the steps of fmx_adjoint_forward (same desc, links, field, tau_field and
- * inlet_series; f/g/q/phi of bufs are not used).  Segment j is recomputed from ckpt slot j into
- * seg, then its adjoint steps run last to first. */
+                        int32_t k, int cur, void *ckpt, int ckpt_prec, void *stream);
+/* Reverse sweep over the steps of fmx_adjoint_forward (same desc, links, field, tau_field,
+ * inlet_series and ckpt_prec; f/g/q/phi of bufs are not used).  Segment j is recomputed from ckpt
+ * slot j into seg, then its adjoint steps run last to first. */
 int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
-                         int32_t k, const void *ckpt, void *seg, const fmx_adjoint_bufs *adj, void *stream);
+                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, const fmx_adjoint_bufs *adj,
+                         void *stream);
 
 /* ---------------------------------------------------------- diagnostics */
 /* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);

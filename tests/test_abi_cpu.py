@@ -67,9 +67,21 @@ def test_adjoint_state_bytes_and_argument_checks():
     a.lam_q = a.lam_phi = a.iso_acc = a.grad_iso = 0x1000
 
     def backward(desc, k=5):
-        return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, 0x1000, C.byref(a), None)
+        return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, desc.prec, 0x1000, C.byref(a), None)
 
+    # 16-bit checkpoints only for an FP32 run
     d = cfg.desc(abi.F32)
+    for cp, s in ((abi.F32, 4), (abi.F16, 2), (abi.BF16, 2)):
+        assert L.fmx_adjoint_ckpt_bytes(C.byref(d), cp) == al(19 * n * s) + al(7 * n * s) + 2 * al(n * 4)
+    assert L.fmx_adjoint_ckpt_bytes(C.byref(d), abi.F64) == -1
+    d64 = cfg.desc(abi.F64)
+    assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F16) == -1
+    assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F64) == L.fmx_adjoint_state_bytes(C.byref(d64))
+    rc = L.fmx_adjoint_backward(C.byref(d64), C.byref(b), 1, 10, 5, 0x1000, abi.F16, 0x1000, C.byref(a), None)
+    assert rc != 0 and b"checkpoint precision" in L.fmx_last_error()
+    # a 16-bit checkpoint store needs a segment store even for k = 1
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 10, 1, 0x1000, abi.F16, None, C.byref(a), None)
+    assert rc != 0 and b"segment store" in L.fmx_last_error()
     d.ns_inlet = 1
     assert backward(d) != 0 and b"NEQ" in L.fmx_last_error()
     d.ns_inlet = 2

@@ -9,6 +9,7 @@ concentration, checked against
     update) in FP64 (relative 1e-6),
   * the FP64 adjoint, for the FP32 adjoint (relative 1e-3),
   * itself under a different checkpoint interval (relative 1e-12),
+  * the FP32-checkpoint adjoint, for an FP32 run with 16-bit checkpoints and FP32 recompute,
 and used through the torch.autograd.Function wrapper.
 """
 import dataclasses
@@ -65,10 +66,11 @@ def adj_config(op="pbb", series=True, closed=False, seed=21, dims=(16, 10, 10), 
 class Problem:
     """Loss, adjoint gradients and finite differences for one configuration."""
 
-    def __init__(self, cfg, prec="f64", nsteps=NSTEPS, k=K, target=None):
-        say(f"adjoint problem prec={prec} op={cfg.ns_op} dims={(cfg.nx, cfg.ny, cfg.nz)} steps={nsteps} k={k}")
+    def __init__(self, cfg, prec="f64", nsteps=NSTEPS, k=K, target=None, ckpt=None):
+        say(f"adjoint problem prec={prec} ckpt={ckpt or prec} op={cfg.ns_op} dims={(cfg.nx, cfg.ny, cfg.nz)} "
+            f"steps={nsteps} k={k}")
         self.cfg = cfg
-        self.run = AdjointRun(cfg, nsteps, k, prec)
+        self.run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt)
         self.base = dict(iso_a=cfg.iso_a, iso_b=cfg.iso_b,
                          field=None if cfg.field is None else np.asarray(cfg.field, np.float64).copy(),
                          series=None if cfg.inlet_series is None else np.asarray(cfg.inlet_series, np.float64).copy())
@@ -388,30 +390,55 @@ def interpolation_matrix(n_knots, steps):
     return B
 
 
+@pytest.mark.parametrize("ckpt,k", [("f16", K), ("bf16", K), ("f16", 1)])
+def test_half_checkpoints_of_fp32_run(pbb_problem, ckpt, k):
+    """An FP32 run whose checkpoints are stored as 16-bit deviations and decoded for an FP32 recompute:
+    the forward curve is the FP32 run's own, bit for bit, and the gradients stay within the 16-bit
+    rounding of the checkpointed states of the FP32-checkpoint adjoint (k = 1 rounds every state)."""
+    p32 = Problem(adj_config("pbb"), prec="f32", target=pbb_problem.target)
+    g32 = p32.grads()
+    ph = Problem(adj_config("pbb"), prec="f32", k=k, target=pbb_problem.target, ckpt=ckpt)
+    assert np.array_equal(ph.c0, p32.c0)
+    gh = ph.grads()
+    g64 = pbb_problem.grads()
+    errs = {name: rel(gh[name], g32[name]) for name in ("iso_a", "iso_b")}
+    errs.update({name: rel_l2(gh[name], g32[name]) for name in ("field", "series")})
+    e64 = {name: rel(gh[name], g64[name]) for name in ("iso_a", "iso_b")}
+    e64.update({name: rel_l2(gh[name], g64[name]) for name in ("field", "series")})
+    say(f"  f32 run, {ckpt} checkpoints (k={k}) vs f32 checkpoints:", {n: f"{v:.2e}" for n, v in errs.items()})
+    say(f"  f32 run, {ckpt} checkpoints (k={k}) vs f64 adjoint:", {n: f"{v:.2e}" for n, v in e64.items()})
+    bound = 1e-3 if ckpt == "bf16" or k == 1 else 1e-4
+    assert max(errs.values()) < bound, errs
+
+
 def test_c5_half_checkpoints_against_fp64():
     """C5 physics (overlapping sphere pack at porosity 0.4, partial bounce-back, Langmuir, 200-knot inlet
-    series) on a 96x64x64 box for 8000 steps: the FP16-stored adjoint with checkpoints every 500 steps
-    (FP32 recompute) against the FP64 adjoint (checkpoints every 200); gradients with respect to the 200
-    knots (chain rule through the per-step interpolation) and the per-cell solid fraction.  As for C4,
-    the fluid cells' solid fraction and the concentration clamp are set to 0 so that the loss has a
-    breakthrough curve to differentiate (see c4_config); the box is shortened so that the front reaches
-    the outlet within the run."""
-    steps = 8000
-    cfg = workloads.build("C5", dims=(96, 64, 64), steps=steps)
+    series), 20000 steps, on a 48x64x64 box (the FP64 reference adjoint's store stays at a few GB): the
+    FP32 run with FP16-stored checkpoints every 500 steps and FP32 recompute against the FP64 adjoint
+    (checkpoints every 200); gradients with respect to the 200 knots (chain rule through the per-step
+    interpolation) and the per-cell solid fraction, errors reported.  As for C4, the fluid cells'
+    solid fraction and the concentration clamp are set to 0.  With the equilibrium inlet holding the
+    inlet density at rho0 the flow through this dense pack is weak and C_out stays small (~1e-4), so
+    the loss is taken against 0.85 x the FP64 curve."""
+    steps = 20000
+    cfg = workloads.build("C5", dims=(48, 64, 64), steps=steps)
     cfg.field = np.where(cfg.field > 0.5, cfg.field, 0.0)
     cfg.min_conc = 0.0
     knots = cfg.meta["knots"]
     B = interpolation_matrix(len(knots), steps)
     assert np.allclose(B @ knots, cfg.inlet_series)
     p64 = Problem(cfg, prec="f64", nsteps=steps, k=200)
-    say(f"  C5 96x64x64 c_out at 2000/4000/6000/8000: {[float('%.3e' % p64.c0[s - 1]) for s in (2000, 4000, 6000, 8000)]}")
-    assert p64.c0[-1] > 1e-3
+    say(f"  C5 48x64x64 c_out at 5000/10000/15000/20000: "
+        f"{[float('%.3e' % p64.c0[s - 1]) for s in (5000, 10000, 15000, 20000)]}")
+    assert p64.c0[-1] > 1e-6
     p64.target = 0.85 * p64.c0
     g64 = p64.grads()
-    p16 = Problem(cfg, prec="f16", nsteps=steps, k=500, target=p64.target)
+    p16 = Problem(cfg, prec="f32", nsteps=steps, k=500, target=p64.target, ckpt="f16")
     g16 = p16.grads()
     k64, k16 = B.T @ g64["series"], B.T @ g16["series"]
-    errs = dict(knots=rel_l2(k16, k64), field=rel_l2(g16["field"], g64["field"]), c_out=rel_l2(p16.c0, p64.c0))
-    say(f"  C5 f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
+    errs = dict(knots=rel_l2(k16, k64), field=rel_l2(g16["field"], g64["field"]), c_out=rel_l2(p16.c0, p64.c0),
+                iso_a=rel(g16["iso_a"], g64["iso_a"]), iso_b=rel(g16["iso_b"], g64["iso_b"]))
+    say(f"  C5 f32 run + f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
         f"|dL/dknots| {np.linalg.norm(k64):.3e}")
-    assert errs["knots"] < 5e-2 and errs["field"] < 1e-1, errs
+    assert errs["c_out"] < 1e-3 and errs["knots"] < 3e-3 and errs["field"] < 3e-3, errs
+    assert errs["iso_a"] < 3e-3 and errs["iso_b"] < 3e-3, errs
