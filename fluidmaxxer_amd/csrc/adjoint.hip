@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                     // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
                     // on a Neumann outlet plane), as the forward step stores them
                     const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                    gi[q] = w7<C>(q) * (p.phi[sref] + widen(p.g[i64(q) * n + s]));
+                    gi[q] = decode<P>(p.g[i64(q) * n + s], w7<C>(q), p.phi[sref]);
                 } else {
                     gi[q] = p.g[i64(q) * n + s];
                 }

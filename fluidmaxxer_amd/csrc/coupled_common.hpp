@@ -67,26 +67,39 @@ template <class C> __device__ __forceinline__ void block_coords(const CoupledPar
 }
 
 // Population plane q of a structure-of-arrays buffer (wave-uniform pointer) and a load /
-// store at a 32-bit byte offset inside it, so the address is a scalar base plus a vector
-// offset (the host guarantees every plane is below 4 GiB).
+// store at a byte offset from it: with a 32-bit unsigned offset the address is a scalar base plus
+// a vector offset (the host guarantees every plane is below 4 GiB).
 template <class T> __device__ __forceinline__ T *plane(T *base, int q, i64 n) { return base + i64(q) * n; }
-template <class T> __device__ __forceinline__ T ld_b(const T *base, unsigned boff) {
+template <class T, class O> __device__ __forceinline__ T ld_b(const T *base, O boff) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
-template <class T> __device__ __forceinline__ T ld_b_nt(const T *base, unsigned boff) {
-    return __builtin_nontemporal_load(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
-}
-template <class T> __device__ __forceinline__ void st_b(T *base, unsigned boff, T v) {
+template <class T, class O> __device__ __forceinline__ void st_b(T *base, O boff, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
 }
-template <class T> __device__ __forceinline__ void st_b_nt(T *base, unsigned boff, T v) {
+template <class T, class O> __device__ __forceinline__ void st_b_nt(T *base, O boff, T v) {
     __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
+}
+// Non-temporal store into a population plane whose base is wave-uniform.  With a 32-bit offset the
+// base and the offset go through empty asm statements (base in scalar registers, one pair per
+// plane; offset a 32-bit value of the store's own block): otherwise the compiler derives every
+// plane's address from the first one's in vector registers, a 64-bit vector add per store, instead
+// of using the scalar-base store.
+template <class T, class O> __device__ __forceinline__ void st_plane_nt(T *base, O boff, T v) {
+    if constexpr (std::is_same_v<O, unsigned>) {
+        asm("" : "+s"(base));
+        asm("" : "+v"(boff));
+        using G = __attribute__((address_space(1))) T;
+        using GC = __attribute__((address_space(1))) char;
+        __builtin_nontemporal_store(v, (G *)((GC *)(G *)base + boff));
+    } else {
+        st_b_nt(base, boff, v);
+    }
 }
 
 // A stored 16-bit value as float.  FP16 is widened by moving its exponent and mantissa bits into
 // an FP32 pattern and scaling by 2^112 (exact for normal and subnormal halves; FP32 subnormals are
-// kept in this build), which issues faster than the hardware conversion on gfx950; BF16 is the top
-// half of an FP32.
+// kept in this build), which runs faster in this kernel on gfx950 than the hardware conversion or a
+// shorter sign-extending form (measured); BF16 is the top half of an FP32.
 template <class S> __device__ __forceinline__ float widen(S v) {
     if constexpr (std::is_same_v<S, _Float16>) {
         unsigned short h;
@@ -98,6 +111,7 @@ template <class S> __device__ __forceinline__ float widen(S v) {
     }
 }
 
+// w (ref + h) for a stored deviation h
 template <int P>
 __device__ __forceinline__ typename Storage<P>::C decode(typename Storage<P>::S v, typename Storage<P>::C w,
                                                           typename Storage<P>::C ref) {
@@ -141,7 +155,8 @@ inline LaunchShape launch_shape(int nx) {
     s.by = 256 / s.bx;
     if (const char *e = std::getenv("FMX_BLOCK")) {
         int a = 0, b = 0;
-        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 256 && (a * b) % 64 == 0) {
+        // rows of whole waves: the kernel takes y as wave-uniform
+        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 256 && a % 64 == 0) {
             s.bx = a;
             s.by = b;
         }

@@ -10,71 +10,78 @@
 
 namespace fmx {
 
-template <int P, int OP, bool NTL>
+// Byte offsets are 32-bit (a scalar plane base plus a vector offset) unless four population planes
+// exceed 4 GiB (WIDE: 64-bit offsets).
+template <int P, int OP, bool WIDE>
 __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 acc_row) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
+    using Off = std::conditional_t<WIDE, i64, unsigned>;
     constexpr unsigned SB = sizeof(S);
     constexpr unsigned CB = sizeof(C);
     int bx, by, bz;
     block_coords(prm, bx, by, bz);
     const int x = bx * prm.block_x + int(threadIdx.x);
-    const int y = by * prm.block_y + int(threadIdx.y);
+    // block_x is a multiple of 64, so every wave lies in one row and y is wave-uniform
+    const int y = __builtin_amdgcn_readfirstlane(by * prm.block_y + int(threadIdx.y));
     const int z = bz;
     if (x >= prm.nx || y >= prm.ny) return;
     const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
     const i64 n = prm.n;
-    const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
-    const unsigned cell_b = cell * SB;
+    const Off cell = Off(x) + Off(nx) * (Off(y) + Off(ny) * Off(z));
+    const Off cell_b = cell * SB;
+    const Off plane_b = Off(n) * SB;
     const std::uint32_t lk = ld_b(p.links, cell * 4u);
     const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
     // a frozen solid cell alternates between the two states held in f and f_new and has zero
     // post-collision velocity in both: no NS loads, collision or stores
     const bool frozen = (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
 
-    // neighbour rows: x keeps local out of domain, y/z wrap when periodic
+    // Source of the pull along (cx, cy, cz): the cell - c, or the cell itself when that lies outside
+    // the box (x is never periodic; y / z wrap when periodic).  The row part is wave-uniform: a scalar
+    // byte delta per direction, applied to this cell's offset; only x = 0 / nx - 1 lanes keep local
+    // for the x-directions.
     const bool xm_ok = x > 0, xp_ok = x < nx - 1;
-    int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
-    bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
+    // rows y - cy and z - cz (index cy + 1, cz + 1) relative to this one, and whether they exist
+    int dy[3] = {1, 0, -1}, dz[3] = {1, 0, -1};
+    bool yin[3] = {y + 1 < ny, true, y > 0}, zin[3] = {z + 1 < nz, true, z > 0};
     if (prm.py) {
-        if (!ym_ok) ym += ny;
-        if (!yp_ok) yp -= ny;
-        ym_ok = yp_ok = true;
+        if (!yin[0]) dy[0] = 1 - ny;
+        if (!yin[2]) dy[2] = ny - 1;
+        yin[0] = yin[2] = true;
     }
     if (prm.pz) {
-        if (!zm_ok) zm += nz;
-        if (!zp_ok) zp -= nz;
-        zm_ok = zp_ok = true;
+        if (!zin[0]) dz[0] = 1 - nz;
+        if (!zin[2]) dz[2] = nz - 1;
+        zin[0] = zin[2] = true;
     }
-    const unsigned ys[3] = {unsigned(ym), unsigned(y), unsigned(yp)};
-    const unsigned zs[3] = {unsigned(zm), unsigned(z), unsigned(zp)};
-    const bool yok[3] = {ym_ok, true, yp_ok};
-    const bool zok[3] = {zm_ok, true, zp_ok};
-    const bool xok[3] = {xm_ok, true, xp_ok};
-    // source cell of the pull along (cx, cy, cz) = cell - c, or the cell itself when out of domain
-    auto src_of = [&](int cx, int cy, int cz) -> unsigned {
-        const bool ok = xok[1 - cx] && yok[1 - cy] && zok[1 - cz];
-        return ok ? unsigned(x - cx) + unsigned(nx) * (ys[1 - cy] + unsigned(ny) * zs[1 - cz]) : cell;
+    auto src_b = [&](int cx, int cy, int cz) -> Off {  // byte offset of the source cell within a plane
+        const bool ok = yin[cy + 1] && zin[cz + 1];
+        const Off r = Off(dy[cy + 1]) * Off(nx) + Off(dz[cz + 1]) * (Off(nx) * Off(ny));
+        const Off d = ok ? (r - Off(cx)) * Off(SB) : Off(0);
+        if (cx == 0) return cell_b + d;
+        return cell_b + (((cx == 1) ? xm_ok : xp_ok) ? d : Off(0));
     };
 
     // ---------------------------------------------------------------- NS pull
+    // With bounce-back links, direction q and its opposite are addressed from the lower of the two
+    // planes, so the bounce-back pull (this cell along opp(q)) and the streaming pull differ only in
+    // the offset, chosen by the link bit.
     C fo[19];
     if (!frozen) static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
-        const unsigned src = src_of(cx, cy, cz);
-        bool bb = false;
-        if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
-        S raw;
-        if constexpr (NTL) {
-            if (bb) raw = ld_b_nt(plane(p.f, Q19::opp[q], n), cell_b);
-            else raw = ld_b_nt(plane(p.f, q, n), src * SB);
-        } else {
-            if (bb) raw = ld_b(plane(p.f, Q19::opp[q], n), cell_b);
-            else raw = ld_b(plane(p.f, q, n), src * SB);
+        constexpr int o = Q19::opp[q];
+        constexpr bool BB = OP == FMX_NS_BGK && q != 0;
+        constexpr int bq = BB ? (q < o ? q : o) : q;
+        Off off = src_b(cx, cy, cz) + Off(q - bq) * plane_b;
+        if constexpr (BB) {
+            const Off boff = cell_b + Off(o - bq) * plane_b;
+            const Off m = Off(int(lk << (31 - q)) >> 31);  // all ones where the q link bounces back
+            off = (boff & m) | (off & ~m);
         }
-        fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
+        fo[q] = decode<P>(ld_b(plane(p.f, bq, n), off), w19<C>(q), prm.f_ref);
     });
 
     // ------------------------------------------- AD pull and sorption inputs
@@ -84,18 +91,19 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
     static_for<0, 7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-        const unsigned s = src_of(cx, cy, cz);
+        const Off s = src_b(cx, cy, cz);
         // x of the source (or keep-local) cell: x - cx when in the domain, else x
         const int sx = (cx == 1) ? (x > 0 ? x - 1 : x) : ((cx == -1) ? (x < nx - 1 ? x + 1 : x) : x);
         if (prm.ad_inlet == 2 && sx == 0) {
             gi[q] = phi_pre * w7<C>(q);
         } else {
-            const S raw = NTL ? ld_b_nt(plane(p.g, q, n), s * SB) : ld_b(plane(p.g, q, n), s * SB);
+            const S raw = ld_b(plane(p.g, q, n), s);
             if constexpr (IsHalf<P>::value) {
                 // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
                 // populations of its x - 1 neighbour, stored about that neighbour's concentration
-                const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                gi[q] = w7<C>(q) * (ld_b(p.phi, sref * CB) + widen(raw));
+                Off sref = s * (CB / SB);
+                if (prm.ad_outlet && sx == nx - 1) sref -= CB;
+                gi[q] = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
             } else {
                 gi[q] = raw;
             }
@@ -150,7 +158,7 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
             C v = fo[q];
             if (inlet_cell) v = (prm.ns_inlet == 2) ? inlet_eq<q>(prm) : inlet_neq<q>(fp[q], rp, up_x, up_y, up_z, prm);
             else if (outlet_cell) v = outlet_pressure<q>(um_x, um_y, um_z, prm);
-            st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+            st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
         });
     } else {
         // the x = 1 thread writes the NEQ inlet of x = 0, the x = nx-2 thread the pressure outlet
@@ -158,13 +166,13 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
             if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
                 static_for<0, 19>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
+                    st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
                 });
             }
         } else if (!outlet_cell && !frozen) {
 #pragma unroll
             for (int q = 0; q < 19; ++q)
-                st_b_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
+                st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
         }
         if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
             static_for<0, 19>([&](auto qc) {
@@ -216,16 +224,16 @@ __global__ void __launch_bounds__(256) k_coupled(CoupledPtrs<P> p, CoupledParams
                 v = gm[q];
                 ref = phim;
             }
-            st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(v, C(Q7::inv_w[q]), ref));
+            st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(v, C(Q7::inv_w[q]), ref));
         });
     } else {
         if (ad_in_cell) {
 #pragma unroll
             for (int q = 0; q < 7; ++q)
-                st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
+                st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
         } else if (!ad_out_cell) {
 #pragma unroll
-            for (int q = 0; q < 7; ++q) st_b_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+            for (int q = 0; q < 7; ++q) st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
         }
         if (prm.ad_outlet && x == nx - 2) {
 #pragma unroll
@@ -249,23 +257,23 @@ int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParam
         if (prm.pair) return launch_coupled_pair<P>(ns_op, ptr, prm, step, acc_row, s);
     }
     const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
-    // FMX_NT_LOADS=1 streams the population pulls with non-temporal loads (measured slower; kept for study)
-    static const bool nt_loads = [] {
-        const char *e = std::getenv("FMX_NT_LOADS");
-        return e != nullptr && std::atoi(e) != 0;
-    }();
-#define FMX_LAUNCH(OP, NT) hipLaunchKernelGGL((k_coupled<P, OP, NT>), grid, block, 0, s, ptr, prm, step, acc_row)
+    // 32-bit offsets reach four population planes from a plane base (a bounce-back pair is at most
+    // three planes apart)
+    // (FMX_WIDE=1 forces the 64-bit form, to check it against the 32-bit one on small boxes)
+    const char *fw = std::getenv("FMX_WIDE");
+    const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(typename Storage<P>::S)) > (i64(1) << 32);
+#define FMX_LAUNCH(OP, W) hipLaunchKernelGGL((k_coupled<P, OP, W>), grid, block, 0, s, ptr, prm, step, acc_row)
     switch (ns_op) {
         case FMX_NS_BGK:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_BGK, true);
+            if (wide) FMX_LAUNCH(FMX_NS_BGK, true);
             else FMX_LAUNCH(FMX_NS_BGK, false);
             break;
         case FMX_NS_POROUS:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_POROUS, true);
+            if (wide) FMX_LAUNCH(FMX_NS_POROUS, true);
             else FMX_LAUNCH(FMX_NS_POROUS, false);
             break;
         case FMX_NS_PBB:
-            if (nt_loads) FMX_LAUNCH(FMX_NS_PBB, true);
+            if (wide) FMX_LAUNCH(FMX_NS_PBB, true);
             else FMX_LAUNCH(FMX_NS_PBB, false);
             break;
         default: return set_error(FMX_ERR_ARG, "fmx_coupled: unknown ns_op %d", ns_op);

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_coupled_pair(CoupledPtrs<P> p, CoupledP
                 // deviation about the source cell's lagged concentration (x - 1 cell's on a Neumann outlet)
                 const unsigned s = in ? src_row + unsigned(xc - cx) : own_row + unsigned(xc);
                 const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                return w7<C>(q) * (p.phi[sref] + widen(raw));
+                return decode<P>(raw, w7<C>(q), p.phi[sref]);
             } else {
                 return C(raw);
             }

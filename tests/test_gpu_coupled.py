@@ -147,6 +147,39 @@ def test_pair_kernel_equals_single_cell_kernel(prec, variant, monkeypatch):
         assert all(v < 5e-3 for v in errs.values()), errs
 
 
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("variant", ["bgk", "pbb"])
+def test_launch_variants_bit_identical(prec, variant, monkeypatch):
+    """Other block shapes, tiled block orders, no XCD remap and 64-bit offsets address the same cells
+    with the same arithmetic: the state after 60 steps is the same to the bit (the outlet sums are FP64
+    atomics, whose order follows the launch)."""
+    import torch
+
+    cfg = small_config(variant, dims=(256, 16, 12))
+    if variant == "bgk":
+        cfg.ns_inlet = "eq"
+    shapes = [{}, {"FMX_BLOCK": "64x4"}, {"FMX_BLOCK": "128x2"}, {"FMX_TILE": "4x6"}, {"FMX_TILE": "16x3"},
+              {"FMX_XCD_REMAP": "0"}, {"FMX_WIDE": "1"}]
+    out = []
+    for env in shapes:
+        for k in ("FMX_BLOCK", "FMX_TILE", "FMX_XCD_REMAP", "FMX_WIDE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        say(f"launch variant {env or 'default'} prec={prec} op={variant}")
+        sim = CoupledSim(cfg, prec=prec)
+        f, g = perturbed_state(cfg, 5, True)
+        sim.load(f=f, g=g)
+        c = sim.run(60)
+        sim.synchronize()
+        out.append(dict(f=sim.f[sim.cur].clone(), g=sim.g[sim.cur].clone(), phi=sim.phi.clone(), q=sim.q.clone(),
+                        c_out=c.clone()))
+    for env, o in zip(shapes[1:], out[1:]):
+        for name in ("f", "g", "phi", "q"):
+            assert torch.equal(o[name], out[0][name]), (env, name)
+        assert torch.allclose(o["c_out"], out[0]["c_out"], rtol=1e-12, atol=0), env
+
+
 @pytest.mark.parametrize("variant", ["bgk", "eq_inlet", "langmuir_series", "pbb"])
 def test_fused_step_boundary_writer_fallback(variant):
     """nx = 65: the x = nx-2 and x = nx-1 cells fall in different waves, so the neighbouring
