@@ -7,8 +7,9 @@ Langmuir LDF, 10000 steps.  Prints ONE JSON line with MLUPS (all cells, solids
 included), the HBM roofline of the fused kernel (algorithmic bytes per cell
 update x cells / measured step time, against 8 TB/s), the adjoint/forward time
 ratio of the checkpointed reverse sweep on the differentiation workload (C4,
-128x64x64 partial bounce-back, 4000 steps, checkpoint every 200) and the CPU
-oracle timed serially on a bounded sample of the same workload.
+128x64x64 partial bounce-back, 4000 steps, checkpoint every 200), the 512^3
+Langmuir pack (C3) timed per storage precision, and the CPU oracle timed
+serially on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -160,6 +161,38 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None):
     return res
 
 
+def large_box_timing(workload="C3", precs=("f32", "f16", "bf16"), steps=100, warmup=20):
+    """The 512^3 Langmuir sphere pack (C3) for each storage precision: step time over `steps` launches
+    between two events on the launch stream, MLUPS and the fraction of the algorithmic roofline."""
+    import torch
+
+    from fluidmaxxer_amd import workloads
+    from fluidmaxxer_amd.coupled import CoupledSim
+
+    cfg = workloads.build(workload, steps=steps + warmup + 1)
+    out = {"workload": f"{workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, "
+                       f"porosity {cfg.meta['porosity']:.4f}, {steps} steps after {warmup}"}
+    for prec in precs:
+        sim = CoupledSim(cfg, prec=prec)
+        sim.run(warmup, log_outlet=True)
+        acc = sim.outlet_acc(steps)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(sim.stream)
+        sim.launch_only(steps, acc=acc)
+        e1.record(sim.stream)
+        torch.cuda.synchronize()
+        ms_step = e0.elapsed_time(e1) / steps
+        achieved = B_ALG[prec] * cfg.n / (ms_step / 1e3) / 1e9
+        out[prec] = {"mlups": round(cfg.n / (ms_step / 1e3) / 1e6, 1), "ms_per_step": round(ms_step, 4),
+                     "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
+        del sim, acc
+        torch.cuda.empty_cache()
+    if "f32" in out and "f16" in out:
+        out["f16_over_f32"] = round(out["f16"]["mlups"] / out["f32"]["mlups"], 3)
+    return out
+
+
 def traffic_from_profile(workload, prec):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload.lower()}_{prec}.json")
     if not os.path.exists(path):
@@ -186,6 +219,7 @@ def main():
                     help="instead of the benchmark, measure plain streaming bandwidth (copy / read / write)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
+    ap.add_argument("--no-large-box", action="store_true", help="skip the 512^3 (C3) timing per storage precision")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -287,10 +321,12 @@ def main():
             "kernel": "k_coupled (one launch per step)",
         },
     }
+    del sim
+    torch.cuda.empty_cache()
     if not args.no_adjoint and args.prec in ("f32", "f64"):
-        del sim
-        torch.cuda.empty_cache()
         line["adjoint"] = adjoint_timing(prec=args.prec)
+    if not args.no_large_box and args.workload != "C3" and dims is None:
+        line["large_box"] = large_box_timing()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -146,6 +146,7 @@ template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy
 // so no lane idles when nx is 64 or 128.  FMX_BLOCK=AxB and FMX_XCD_REMAP=0|1 override.
 struct LaunchShape {
     int bx = 256, by = 1, remap = 1, ty = 0, tz = 0;
+    bool tiled = false;  // FMX_TILE given: no default slab order
 };
 
 inline LaunchShape launch_shape(int nx) {
@@ -168,6 +169,7 @@ inline LaunchShape launch_shape(int nx) {
             s.ty = a;
             s.tz = b;
         }
+        s.tiled = true;  // FMX_TILE=none (or any non-tile value): the plain y-then-z order
     }
     return s;
 }
@@ -196,6 +198,18 @@ template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
     if (ls.ty > 0 && ls.tz > 0 && p.gy % ls.ty == 0 && p.gz % ls.tz == 0) {
         p.tile_y = ls.ty;
         p.tile_z = ls.tz;
+    } else if (!ls.tiled && p.gx * p.gy > 256) {
+        // Large z-planes: slabs of block rows in y running through the whole z extent, so the rows a
+        // block shares bounce-back lines with (z +- 1) come about 128 blocks later in the launch
+        // order instead of a whole y-plane of blocks later, while each z-plane of a slab stays a run
+        // of contiguous rows (512^3 FP32: +12.6%; no gain where a z-plane is 128 blocks)
+        int ty = 1;
+        for (int t = 1; t <= p.gy; ++t)
+            if (p.gy % t == 0 && t * p.gx <= 128) ty = t;
+        if (ty < p.gy) {
+            p.tile_y = ty;
+            p.tile_z = p.gz;
+        }
     }
     // FMX_PAIR=1 selects the two-cells-per-thread kernel where nx % 128 == 0 (measured no faster for
     // FP32 and slower for FP16/BF16 storage, so off by default); the frozen-solid path (FMX_FREEZE=1)

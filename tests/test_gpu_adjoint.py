@@ -296,6 +296,44 @@ def c4_config(steps):
     return cfg
 
 
+def test_c4_recovers_isotherm_constants():
+    """The C4 inverse problem as specified: the target curve is the forward run with q_max x 1.1 and
+    K_L x 0.9, so the minimum of L = sum_s (C_out(s) - C*(s))^2 is known (0, at those constants).
+    L-BFGS on (log q_max, log K_L), through the torch.autograd.Function and the FP32 adjoint
+    (checkpoints every 200 steps), recovers both constants."""
+    import math
+
+    import torch
+
+    steps = 12000
+    cfg = c4_config(steps)
+    truth = (cfg.iso_a * 1.1, cfg.iso_b * 0.9)
+    trun = AdjointRun(dataclasses.replace(cfg, iso_a=truth[0], iso_b=truth[1]), steps, 200, "f32")
+    target = trun.forward().clone()
+    del trun
+    run = AdjointRun(cfg, steps, 200, "f32")
+    theta = torch.tensor([math.log(cfg.iso_a), math.log(cfg.iso_b)], dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([theta], lr=1.0, max_iter=25, line_search_fn="strong_wolfe", tolerance_grad=1e-14,
+                            tolerance_change=1e-20)
+    scale = float((target ** 2).sum())  # C_out stays below ~1e-3 here: the loss relative to the curve
+    hist = []
+
+    def closure():
+        opt.zero_grad()
+        c = CoupledBreakthrough.apply(run, theta[0].exp(), theta[1].exp(), None, None)
+        loss = ((c - target) ** 2).sum() / scale
+        loss.backward()
+        hist.append(float(loss))
+        return loss
+
+    opt.step(closure)
+    q_max, k_l = (float(v) for v in theta.detach().exp())
+    say(f"  C4 recovery: {len(hist)} evaluations, loss {hist[0]:.3e} -> {min(hist):.3e}; q_max {q_max:.5f} "
+        f"(truth {truth[0]:.5f}), K_L {k_l:.5f} (truth {truth[1]:.5f})")
+    assert min(hist) < 1e-8 * hist[0], hist
+    assert abs(q_max / truth[0] - 1) < 1e-3 and abs(k_l / truth[1] - 1) < 1e-3, (q_max, k_l, truth)
+
+
 def test_c4_differentiation_fp64_vs_fd():
     """C4-scale FP64 adjoint (12000 steps, checkpoint every 200; target = the forward run with
     q_max x 1.1 and K_L x 0.9) against FP64 central differences of the GPU forward run."""
