@@ -147,6 +147,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FmxError(f"{LIB_PATH} is missing: build it with `python -m fluidmaxxer_amd.build`")
+        # torch's HIP runtime first: its libamdhip64 carries the soname this library links against, so
+        # loaded first it serves both.  Loaded after this library, torch brings a second HIP / HSA
+        # runtime into the process and this library's first launch finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name)
