@@ -230,7 +230,10 @@ class CoupledSim:
         def dev(a):
             if a is None:
                 return None
-            t = torch.as_tensor(np.asarray(a)).to(device=self.device, dtype=self.cdt).contiguous()
+            if isinstance(a, torch.Tensor):  # device tensors stay on the device
+                t = a.reshape(-1).to(device=self.device, dtype=self.cdt).contiguous()
+            else:
+                t = torch.as_tensor(np.asarray(a)).to(device=self.device, dtype=self.cdt).contiguous()
             keep.append(t)
             return t.data_ptr()
 

@@ -1,0 +1,167 @@
+"""The fused step at the workloads' own sizes: C2 (256x128x128) against the oracle, and the 512^3
+pack (C3, 134M cells: 32-bit plane offsets up to 2 GiB, the y-slab block order) through properties
+that hold at any size -- the result does not depend on the block order or the offset width, FP32
+stays within FP32 rounding of FP64, 16-bit storage within its rounding of FP32, and a box closed
+by bounce-back walls conserves its fluid mass."""
+import numpy as np
+import pytest
+
+from fluidmaxxer_amd import workloads
+from fluidmaxxer_amd.coupled import CoupledSim
+from oracle import ref_cpu as R
+from tests.gpu_util import host, say
+from tests.lattice import W19, W7, aos_to_soa, oracle_setup, rel_err, soa_to_aos
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def perturbed_device_state(n, rho0, phi0, seed):
+    """f = w rho0 (1 + 0.01 xi), g = w max(phi0, 0.05) (1 + 0.01 xi), xi standard normal, FP32-rounded,
+    generated on the device (19 x 134M values at 512^3)."""
+    torch = _torch()
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    w19 = torch.tensor(W19, dtype=torch.float64, device="cuda").reshape(19, 1)
+    w7 = torch.tensor(W7, dtype=torch.float64, device="cuda").reshape(7, 1)
+    f = (w19 * rho0 * (1 + 0.01 * torch.randn(19, n, generator=gen, device="cuda", dtype=torch.float32))).float()
+    g = (w7 * max(phi0, 0.05) * (1 + 0.01 * torch.randn(7, n, generator=gen, device="cuda", dtype=torch.float32))).float()
+    return f, g
+
+
+def test_c2_matches_oracle_bitwise():
+    """C2 as specified (256x128x128 overlapping pack, Langmuir, equilibrium inlet), from a perturbed
+    state, 10 steps: rho, u, phi, q and g equal the FP32 oracle's to the bit."""
+    cfg = workloads.build("C2", steps=20)
+    say(f"C2 vs oracle, {cfg.n} cells, 10 steps")
+    rng = np.random.default_rng(3)
+    f = (np.outer(W19, np.full(cfg.n, cfg.rho0)) * (1 + 0.01 * rng.standard_normal((19, cfg.n)))).astype(np.float32)
+    g = (np.outer(W7, np.full(cfg.n, 0.05)) * (1 + 0.01 * rng.standard_normal((7, cfg.n)))).astype(np.float32)
+    sim = CoupledSim(cfg, prec="f32")
+    sim.load(f=f, g=g)
+    orc = R.System(oracle_setup(cfg, R), 4)
+    orc.set(R.F, soa_to_aos(f.astype(np.float64), 19))
+    orc.set(R.G, soa_to_aos(g.astype(np.float64), 7))
+    outlet = (np.arange(cfg.n) % cfg.nx == cfg.nx - 1).astype(np.uint8)
+    c = host(sim.run(10))
+    cr, _, _, _ = orc.run(10, outlet)
+    rho, ux, uy, uz = (host(t) for t in sim.moments())
+    got = dict(rho=rho, ux=ux, uy=uy, uz=uz, phi=host(sim.phi), q=host(sim.q), g=host(sim.view_g()).ravel())
+    ref = dict(rho=orc.get(R.RHO), ux=orc.get(R.UX), uy=orc.get(R.UY), uz=orc.get(R.UZ), phi=orc.get(R.PHI),
+               q=orc.get(R.Q), g=aos_to_soa(orc.get(R.G), 7).ravel())
+    orc.close()
+    diff = {k: float(np.max(np.abs(got[k] - ref[k]))) for k in got}
+    say("  max |diff|:", {k: f"{v:.1e}" for k, v in diff.items()}, f"c_out {rel_err(c, cr):.1e}")
+    assert all(v == 0.0 for v in diff.values()), diff
+    assert rel_err(c, cr) < 1e-12  # FP64 outlet sums, atomic order
+
+
+@pytest.fixture(scope="module")
+def c3():
+    return workloads.build("C3", steps=200)
+
+
+def _state(sim):
+    return dict(f=sim.f[sim.cur].clone(), g=sim.g[sim.cur].clone(), phi=sim.phi.clone(), q=sim.q.clone())
+
+
+def test_c3_block_order_and_offset_width_bit_identical(c3, monkeypatch):
+    """512^3 FP32, 20 steps from a perturbed state: the default y-slab block order, the plain y-then-z
+    order and 64-bit offsets give the same bits."""
+    torch = _torch()
+    f, g = perturbed_device_state(c3.n, c3.rho0, c3.phi0, 11)
+    ref = None
+    for env in ({}, {"FMX_TILE": "none"}, {"FMX_WIDE": "1"}):
+        for k in ("FMX_TILE", "FMX_WIDE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        say(f"C3 FP32 launch variant {env or 'default'}")
+        sim = CoupledSim(c3, prec="f32")
+        sim.load(f=f, g=g)
+        sim.run(20, log_outlet=False)
+        sim.synchronize()
+        st = _state(sim)
+        del sim
+        if ref is None:
+            ref = st
+            continue
+        for name in ref:
+            assert torch.equal(st[name], ref[name]), (env, name)
+        del st
+        torch.cuda.empty_cache()
+
+
+def test_c3_fp32_against_fp64(c3):
+    """512^3 from the equilibrium start, 100 steps: the FP32 kernel within 5e-5 (relative to each
+    field's scale) of the FP64 kernel, which equals the FP64 oracle to the bit on small boxes.  The
+    bound is FP32's own drift from FP64 (the FP32 oracle drifts 3.9e-5 in u over C1's 1000 steps;
+    1.1e-5 in u here); against the reference FP32 update the kernel agrees to the bit."""
+    torch = _torch()
+    out = {}
+    for prec in ("f64", "f32"):
+        say(f"C3 {prec} 100 steps")
+        sim = CoupledSim(c3, prec=prec)
+        c = sim.run(100)
+        rho, ux = sim.moments()[:2]
+        out[prec] = dict(rho=rho.double(), ux=ux.double(), phi=sim.phi.double(), q=sim.q.double(), c_out=c.double())
+        del sim, rho, ux
+        torch.cuda.empty_cache()
+    errs = {k: float((out["f32"][k] - out["f64"][k]).abs().max() / out["f64"][k].abs().max().clamp_min(1e-300))
+            for k in out["f64"]}
+    say("  f32 vs f64:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(v < 5e-5 for v in errs.values()), errs
+
+
+@pytest.mark.parametrize("prec,bound", [("f16", 1e-3), ("bf16", 5e-3)])
+def test_c3_half_storage_against_fp32(c3, prec, bound):
+    """512^3, 100 steps from the equilibrium start: 16-bit deviation storage against FP32 storage,
+    relative to each field's scale; the outlet curve, still at the 1e-8 concentration floor after 100
+    steps, against the inlet concentration (FP16 resolves about 6e-8 near zero)."""
+    torch = _torch()
+    out = {}
+    for p in ("f32", prec):
+        say(f"C3 {p} 100 steps")
+        sim = CoupledSim(c3, prec=p)
+        c = sim.run(100)
+        rho, ux = sim.moments()[:2]
+        out[p] = dict(rho=rho.double(), ux=ux.double(), phi=sim.phi.double(), c_out=c.double())
+        del sim, rho, ux
+        torch.cuda.empty_cache()
+    scale = {k: float(out["f32"][k].abs().max()) for k in out["f32"]}
+    scale["c_out"] = c3.phi_in
+    errs = {k: float((out[prec][k] - out["f32"][k]).abs().max()) / scale[k] for k in out["f32"]}
+    say(f"  {prec} vs f32:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(v < bound for v in errs.values()), errs
+
+
+def test_c3_closed_box_conserves_fluid_mass(c3):
+    """512^3 pack closed by solid x = 0 and x = nx-1 planes besides the y / z walls, no inlet or outlet,
+    perturbed FP64 start, 50 steps: halfway bounce-back keeps the fluid mass (sum of rho over the fluid
+    cells) to FP64 rounding."""
+    torch = _torch()
+    cfg = workloads.build("C3", steps=100)
+    s = cfg.solid.reshape(cfg.nz, cfg.ny, cfg.nx).copy()
+    s[:, :, 0] = s[:, :, -1] = 1
+    cfg.solid = s.ravel()
+    cfg.ns_inlet, cfg.ns_outlet = None, False
+    cfg.ad_inlet, cfg.ad_outlet = "none", False
+    say(f"C3 closed box FP64, fluid fraction {1 - cfg.solid.mean():.4f}")
+    sim = CoupledSim(cfg, prec="f64")
+    f, g = perturbed_device_state(cfg.n, cfg.rho0, cfg.phi0, 12)
+    sim.load(f=f, g=g)
+    del f, g
+    fluid = torch.as_tensor(cfg.solid == 0, device="cuda")
+
+    def fluid_mass():
+        rho = sim.moments()[0]
+        return float(rho[fluid].sum())
+
+    m0 = fluid_mass()
+    sim.run(50, log_outlet=False)
+    m1 = fluid_mass()
+    say(f"  fluid mass {m0:.15e} -> {m1:.15e}, relative change {abs(m1 - m0) / m0:.2e}")
+    assert abs(m1 - m0) / m0 < 1e-12
