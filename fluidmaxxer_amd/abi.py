@@ -121,7 +121,9 @@ _SIGS = {
     "fmx_adjoint_state_bytes": ([_DP], C.c_int64),
     "fmx_adjoint_ckpt_bytes": ([_DP, _i], C.c_int64),
     "fmx_adjoint_forward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _i, _vp, _i, _vp], _i),
-    "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _i, _vp, C.POINTER(AdjointBufs), _vp], _i),
+    "fmx_adjoint_seg_slots": ([C.c_int32, _i, _i], C.c_int64),
+    "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _i, _vp, _i64, C.POINTER(AdjointBufs), _vp],
+                             _i),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
     "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,

@@ -35,7 +35,7 @@ def _torch():
 
 class AdjointRun:
     def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0",
-                 ckpt: str = None):
+                 ckpt: str = None, seg_slots: int = None):
         torch = _torch()
         if prec not in ("f32", "f64", "f16", "bf16"):
             raise abi.FmxError(f"unknown precision {prec}")
@@ -56,10 +56,18 @@ class AdjointRun:
         self.ckpt_prec = abi.PREC_NAMES[ckpt]
         self.ckpt_bytes = int(abi.lib().fmx_adjoint_ckpt_bytes(abi.C.byref(d), self.ckpt_prec))
         nseg = (self.nsteps + self.k - 1) // self.k
-        nslots = self.k if ckpt != prec else max(self.k - 1, 1)
+        kk = min(self.k, self.nsteps)
+        recast = int(ckpt != prec)
+        # segment store: one level by default; fewer slots (at least the two-level minimum, about
+        # 2 sqrt(k)) make the reverse sweep keep sub-checkpoints within each segment
+        single = int(abi.lib().fmx_adjoint_seg_slots(kk, recast, 0))
+        self.seg_slots = single if seg_slots is None else int(seg_slots)
+        least = int(abi.lib().fmx_adjoint_seg_slots(kk, recast, 1))
+        if self.seg_slots < least:
+            raise abi.FmxError(f"segments of {kk} steps need at least {least} segment slots")
         u8 = torch.uint8
         self.ckpt = torch.empty(nseg * self.ckpt_bytes, dtype=u8, device=self.device)
-        self.seg = torch.empty(nslots * self.state_bytes, dtype=u8, device=self.device)
+        self.seg = torch.empty(max(self.seg_slots, 1) * self.state_bytes, dtype=u8, device=self.device)
         self.lam_f = [torch.empty(19 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
         self.lam_g = [torch.empty(7 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
         self.lam_q = torch.empty(n, dtype=self.cdt, device=self.device)
@@ -129,7 +137,7 @@ class AdjointRun:
         a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
         sim = self.sim
         abi.call("fmx_adjoint_backward", sim._desc_p(), sim._bufs_p(), 1, self.nsteps, self.k, self.ckpt.data_ptr(),
-                 self.ckpt_prec, self.seg.data_ptr(), abi.C.byref(a), sim._s())
+                 self.ckpt_prec, self.seg.data_ptr(), self.seg_slots, abi.C.byref(a), sim._s())
         sim.synchronize()
         gi = self.grad_iso.cpu().numpy()
         return {

@@ -655,9 +655,25 @@ static int forward_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i6
     return FMX_OK;
 }
 
+// Segment-store slots a reverse sweep over segments of `k` steps needs: one state per step of the
+// segment after its first (plus the decoded checkpoint when checkpoints are 16-bit), or, two-level,
+// sub-checkpoints every m steps and one sub-segment of recomputed states: (nsub - 1) + (m - 1).
+inline i64 seg_slots_single(int k, bool recast) { return recast ? k : (k > 1 ? k - 1 : 0); }
+inline int sub_length(int len, bool recast, i64 slots) {
+    for (int m = 1; m <= len; ++m) {
+        const int nsub = (len + m - 1) / m;
+        if ((recast ? 1 : 0) + (nsub - 1) + (m - 1) <= slots) return m;
+    }
+    return 0;
+}
+inline i64 seg_slots_min(int k, bool recast) {
+    for (i64 slots = 0;; ++slots)
+        if (sub_length(k, recast, slots) > 0) return slots;
+}
+
 template <int P>
 static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int k,
-                    void *ckpt, int cp, void *seg, const fmx_adjoint_bufs &adj, hipStream_t s) {
+                    void *ckpt, int cp, void *seg, i64 seg_slots, const fmx_adjoint_bufs &adj, hipStream_t s) {
     using C = typename Storage<P>::C;
     const i64 n = i64(d.nx) * d.ny * d.nz;
     const CoupledParams<C> prm = make_params<C>(d);
@@ -676,14 +692,12 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
     for (const Z &zz : zeros)
         if (zz.p && zz.bytes)
             if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
+    const bool recast = cp != P;  // 16-bit checkpoints of an FP32 run: decoded into seg slot 0
     const int nseg = (nsteps + k - 1) / k;
     int lam = 0;
     for (int j = nseg - 1; j >= 0; --j) {
         const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
         const i64 s0 = step0 + i64(j) * k;
-        // states at the start of steps s0 .. s0+len-1: the checkpoint, then seg slots 0 .. len-2; a
-        // 16-bit checkpoint of an FP32 run is decoded into seg slot 0 and the states follow in 1 .. len-1
-        const bool recast = cp != P;
         if constexpr (P == FMX_F32) {
             if (cp == FMX_F16)
                 if (int rc = load_half_ckpt<FMX_F16>(d, state_at<FMX_F16>(ckpt, j, n), state_at<P>(seg, 0, n), n, s))
@@ -692,15 +706,33 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
                 if (int rc = load_half_ckpt<FMX_BF16>(d, state_at<FMX_BF16>(ckpt, j, n), state_at<P>(seg, 0, n), n, s))
                     return rc;
         }
-        auto state = [&](int i) {
-            if (recast) return state_at<P>(seg, i, n);
-            return i == 0 ? state_at<P>(ckpt, j, n) : state_at<P>(seg, i - 1, n);
+        // Sub-segments of m steps (m = len: one level).  Slots: [decoded checkpoint], the states at
+        // the starts of sub-segments 1 .. nsub-1, then the recomputed states 1 .. m-1 of one sub-segment.
+        const int m = seg_slots >= seg_slots_single(len, recast) ? len : sub_length(len, recast, seg_slots);
+        const int nsub = (len + m - 1) / m;
+        const i64 base = recast ? 1 : 0;
+        // state at the start of step s0 + i*m + r of segment j (r < m)
+        auto state = [&](int i, int r) {
+            if (r > 0) return state_at<P>(seg, base + (nsub - 1) + (r - 1), n);
+            if (i > 0) return state_at<P>(seg, base + (i - 1), n);
+            return recast ? state_at<P>(seg, 0, n) : state_at<P>(ckpt, j, n);
         };
-        for (int i = 1; i < len; ++i)
-            if (int rc = step_between<P>(d, b, state(i - 1), state(i), prm, s0 + i - 1, s)) return rc;
-        for (int i = len - 1; i >= 0; --i) {
-            if (int rc = adjoint_step<P>(d, b, state(i), adj, prm, s0 + i, s0 + i - step0, lam, s)) return rc;
-            lam ^= 1;
+        // forward through sub-segments 0 .. nsub-2 to their successors' starting states
+        for (int i = 0; i + 1 < nsub; ++i)
+            for (int r = 1; r <= m; ++r) {
+                const auto dst = r < m ? state(i, r) : state(i + 1, 0);
+                if (int rc = step_between<P>(d, b, state(i, r - 1), dst, prm, s0 + i64(i) * m + r - 1, s)) return rc;
+            }
+        // each sub-segment from the last: recompute its states, then its adjoint steps in reverse
+        for (int i = nsub - 1; i >= 0; --i) {
+            const int mi = (i + 1) * m <= len ? m : len - i * m;
+            const i64 t0 = s0 + i64(i) * m;
+            for (int r = 1; r < mi; ++r)
+                if (int rc = step_between<P>(d, b, state(i, r - 1), state(i, r), prm, t0 + r - 1, s)) return rc;
+            for (int r = mi - 1; r >= 0; --r) {
+                if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t0 + r, t0 + r - step0, lam, s)) return rc;
+                lam ^= 1;
+            }
         }
     }
     hipLaunchKernelGGL(k_iso_final, dim3(1), dim3(64), 0, s, adj.iso_acc, adj.grad_iso);
@@ -751,13 +783,24 @@ int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, in
     FMX_TRY_END
 }
 
+int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level) {
+    if (k < 1) return -1;
+    return two_level ? seg_slots_min(k, recast != 0) : seg_slots_single(k, recast != 0);
+}
+
 int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
-                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, const fmx_adjoint_bufs *adj,
-                         void *stream) {
+                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, int64_t seg_slots,
+                         const fmx_adjoint_bufs *adj, void *stream) {
     FMX_TRY_BEGIN
     if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_backward")) return rc;
-    if (!ckpt || (((k > 1 && nsteps > 1) || ckpt_prec != d->prec) && !seg) || !adj)
-        return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing checkpoint or segment store");
+    const bool recast = ckpt_prec != d->prec;
+    const int kk = nsteps < k ? nsteps : k;
+    const i64 need = seg_slots_min(kk, recast);
+    if (!ckpt || !adj || (need > 0 && (!seg || seg_slots < need)))
+        return set_error(FMX_ERR_ARG,
+                         "fmx_adjoint_backward: missing checkpoint or segment store (segments of %d steps need at "
+                         "least %lld segment slots, %lld for one level; %lld given)",
+                         kk, (long long)need, (long long)seg_slots_single(kk, recast), (long long)seg_slots);
     if (!adj->lam_f[0] || !adj->lam_f[1] || !adj->lam_g[0] || !adj->lam_g[1] || !adj->lam_q || !adj->lam_phi ||
         !adj->iso_acc || !adj->grad_iso)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing adjoint buffer");
@@ -766,10 +809,10 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
     switch (d->prec) {
-        case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
-        case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
-        case FMX_F16: return backward<FMX_F16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
-        default: return backward<FMX_BF16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, *adj, s);
+        case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);
+        case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);
+        case FMX_F16: return backward<FMX_F16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);
+        default: return backward<FMX_BF16>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);
     }
     FMX_TRY_END
 }

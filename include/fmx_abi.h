@@ -237,7 +237,11 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  * Checkpoint precision `ckpt_prec`: the storage precision (desc->prec: states kept as they are), or,
  * for an FP32 run, FMX_F16 / FMX_BF16: f and g are checkpointed as 16-bit deviations (f/w - f_ref,
  * g/w - phi_src, the 16-bit storage layout of the forward step), q and phi in FP32, and each segment
- * is recomputed in FP32 from its decoded checkpoint (seg then holds k slots instead of k-1). */
+ * is recomputed in FP32 from its decoded checkpoint (seg then holds k slots instead of k-1).
+ * Two levels: given fewer segment slots than that (at least fmx_adjoint_seg_slots(k, recast, 1),
+ * about 2 sqrt(k)), the sweep keeps sub-checkpoints every m steps within a segment and recomputes
+ * one sub-segment at a time (one more forward pass over each segment; the same gradients, bit for
+ * bit). */
 #define FMX_ADJ_SLOTS 64
 #define FMX_ADJ_STRIDE 16 /* doubles between accumulator slots */
 
@@ -263,12 +267,16 @@ int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *desc, int ckpt_prec);
  * update path). */
 int fmx_adjoint_forward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
                         int32_t k, int cur, void *ckpt, int ckpt_prec, void *stream);
+/* Segment slots (of fmx_adjoint_state_bytes) a reverse sweep over segments of k steps needs:
+ * one level (two_level = 0) or the two-level minimum (two_level = 1); recast = 1 for 16-bit
+ * checkpoints of an FP32 run.  -1 for k < 1. */
+int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level);
 /* Reverse sweep over the steps of fmx_adjoint_forward (same desc, links, field, tau_field,
  * inlet_series and ckpt_prec; f/g/q/phi of bufs are not used).  Segment j is recomputed from ckpt
- * slot j into seg, then its adjoint steps run last to first. */
+ * slot j into seg (seg_slots state slots), then its adjoint steps run last to first. */
 int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
-                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, const fmx_adjoint_bufs *adj,
-                         void *stream);
+                         int32_t k, const void *ckpt, int ckpt_prec, void *seg, int64_t seg_slots,
+                         const fmx_adjoint_bufs *adj, void *stream);
 
 /* ---------------------------------------------------------- diagnostics */
 /* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);

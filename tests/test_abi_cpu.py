@@ -67,7 +67,8 @@ def test_adjoint_state_bytes_and_argument_checks():
     a.lam_q = a.lam_phi = a.iso_acc = a.grad_iso = 0x1000
 
     def backward(desc, k=5):
-        return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, desc.prec, 0x1000, C.byref(a), None)
+        return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, desc.prec, 0x1000, k,
+                                      C.byref(a), None)
 
     # 16-bit checkpoints only for an FP32 run
     d = cfg.desc(abi.F32)
@@ -77,11 +78,18 @@ def test_adjoint_state_bytes_and_argument_checks():
     d64 = cfg.desc(abi.F64)
     assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F16) == -1
     assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F64) == L.fmx_adjoint_state_bytes(C.byref(d64))
-    rc = L.fmx_adjoint_backward(C.byref(d64), C.byref(b), 1, 10, 5, 0x1000, abi.F16, 0x1000, C.byref(a), None)
+    rc = L.fmx_adjoint_backward(C.byref(d64), C.byref(b), 1, 10, 5, 0x1000, abi.F16, 0x1000, 5, C.byref(a), None)
     assert rc != 0 and b"checkpoint precision" in L.fmx_last_error()
     # a 16-bit checkpoint store needs a segment store even for k = 1
-    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 10, 1, 0x1000, abi.F16, None, C.byref(a), None)
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 10, 1, 0x1000, abi.F16, None, 0, C.byref(a), None)
     assert rc != 0 and b"segment store" in L.fmx_last_error()
+    # segment slots: one level k - 1 (k with 16-bit checkpoints); two levels about 2 sqrt(k)
+    assert L.fmx_adjoint_seg_slots(64, 0, 0) == 63 and L.fmx_adjoint_seg_slots(64, 1, 0) == 64
+    assert L.fmx_adjoint_seg_slots(64, 0, 1) == 14 and L.fmx_adjoint_seg_slots(64, 1, 1) == 15
+    assert L.fmx_adjoint_seg_slots(500, 0, 1) == 43 and L.fmx_adjoint_seg_slots(1, 0, 1) == 0
+    assert L.fmx_adjoint_seg_slots(0, 0, 0) == -1
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 200, 64, 0x1000, abi.F32, 0x1000, 13, C.byref(a), None)
+    assert rc != 0 and b"at least 14 segment slots" in L.fmx_last_error()
     d.ns_inlet = 1
     assert backward(d) != 0 and b"NEQ" in L.fmx_last_error()
     d.ns_inlet = 2

@@ -296,6 +296,23 @@ def c4_config(steps):
     return cfg
 
 
+@pytest.mark.parametrize("prec,ckpt,slots", [("f64", None, 14), ("f64", None, 20), ("f32", "f16", 15)])
+def test_two_level_segments_bit_identical(pbb_problem, prec, ckpt, slots):
+    """A segment store of about 2 sqrt(k) slots (sub-checkpoints within each segment, one more forward
+    pass) gives the one-level sweep's gradients: the recomputed states are the same, so the isotherm
+    and per-cell gradients agree to the bit; the inlet-series gradient sums FP64 block partials with
+    atomics, whose order varies from run to run (1e-12)."""
+    one = Problem(adj_config("pbb"), prec=prec, target=pbb_problem.target, ckpt=ckpt)
+    g1 = one.grads()
+    two = Problem(adj_config("pbb"), prec=prec, target=pbb_problem.target, ckpt=ckpt)
+    two.run = AdjointRun(two.cfg, NSTEPS, K, prec, ckpt=ckpt, seg_slots=slots)
+    say(f"  two-level: {slots} segment slots for k = {K} (one level: {one.run.seg_slots})")
+    g2 = two.grads()
+    assert g1["iso_a"] == g2["iso_a"] and g1["iso_b"] == g2["iso_b"]
+    assert np.array_equal(g1["field"], g2["field"])
+    assert np.allclose(g1["series"], g2["series"], rtol=1e-12, atol=1e-12 * np.abs(g1["series"]).max())
+
+
 def test_c4_recovers_isotherm_constants():
     """The C4 inverse problem as specified: the target curve is the forward run with q_max x 1.1 and
     K_L x 0.9, so the minimum of L = sum_s (C_out(s) - C*(s))^2 is known (0, at those constants).
