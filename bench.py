@@ -110,9 +110,10 @@ def stream_probe(gib=2, reps=10):
     return {"stream_probe": out}
 
 
-def adjoint_timing(workload="C4", prec="f32", k=200, steps=None):
+def adjoint_timing(workload="C4", prec="f32", k=200, steps=None, ckpt=None, two_level=False):
     """Time of the checkpointed reverse sweep (recompute + adjoint steps) against the plain forward
-    run of the same steps, on the differentiation workload (C4: 128x64x64 partial bounce-back)."""
+    run of the same steps, on a differentiation workload (default C4: 128x64x64 partial bounce-back;
+    C5 at 256x128x128 with 16-bit checkpoints every 500 steps and a two-level segment store)."""
     import torch
 
     from fluidmaxxer_amd import abi, workloads
@@ -120,7 +121,10 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None):
 
     nsteps = steps or workloads.SPECS[workload]["steps"]
     cfg = workloads.build(workload, steps=nsteps)
-    run = AdjointRun(cfg, nsteps, k, prec)
+    seg_slots = None
+    if two_level:
+        seg_slots = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+    run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt, seg_slots=seg_slots)
     sim = run.sim
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
 
@@ -149,7 +153,8 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None):
     t_bwd, _ = timed(lambda: run.backward(w))
     res = {
         "workload": f"{workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, {nsteps} steps, "
-                    f"checkpoint every {k}, {prec}",
+                    f"checkpoint every {k} ({ckpt or prec}), {prec} run, {run.seg_slots} segment slots",
+        "store_gib": round((run.ckpt.numel() + run.seg.numel()) / 2 ** 30, 2),
         "forward_ms": round(t_plain, 3),
         "forward_with_checkpoints_ms": round(t_fwd, 3),
         "backward_ms": round(t_bwd, 3),
@@ -220,6 +225,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
     ap.add_argument("--no-large-box", action="store_true", help="skip the 512^3 (C3) timing per storage precision")
+    ap.add_argument("--adjoint-c5", action="store_true",
+                    help="only the adjoint/forward timing on C5 as specified (256x128x128, 20000 steps, FP16 "
+                         "checkpoints every 500 steps, FP32 recompute; one-level and two-level segment stores)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -235,6 +243,13 @@ def main():
 
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
+        return
+
+    if args.adjoint_c5:
+        # one level: 499 FP32 segment slots (about 220 GiB); two levels: about 2 sqrt(500) slots
+        for two_level in (False, True):
+            print(json.dumps({"adjoint": adjoint_timing("C5", "f32", k=500, ckpt="f16", two_level=two_level)}),
+                  flush=True)
         return
 
     if args.adjoint_only:

@@ -51,8 +51,8 @@ template <int P> struct AdjointPtrs {
     C *lam_phi;  // in place: dL/d(concentration written by the step) -> dL/d(lagged concentration)
     double *grad_field;
     double *grad_series;
-    double *iso_acc;
     const double *outlet_weight;
+    double *partials;            // 3 per block: the block's parameter-gradient sums of this step
 };
 
 // Block sum of three values; lane 0 of the block receives the totals.
@@ -411,33 +411,42 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
     }
 
     // ---------------------------------------------------- parameter gradients
-    const bool want_series = prm.ad_inlet == 2 && p.grad_series && bx == 0;
-    if (!prm.isotherm && !want_series) return;  // block-uniform
+    // Every block stores its sums; k_param_sum then adds them in block order, so the gradients repeat
+    // to the bit from run to run.
+    if (!prm.isotherm && !(prm.ad_inlet == 2 && p.grad_series)) return;  // grid-uniform
     block_sum3(acc_a, acc_b, acc_s);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int slot = int(blockIdx.x % FMX_ADJ_SLOTS);
-        if (prm.isotherm) {
-            unsafeAtomicAdd(p.iso_acc + slot * FMX_ADJ_STRIDE, acc_a);
-            unsafeAtomicAdd(p.iso_acc + slot * FMX_ADJ_STRIDE + 1, acc_b);
+        double *pp = p.partials + 3 * i64(blockIdx.x);
+        pp[0] = acc_a;
+        pp[1] = acc_b;
+        pp[2] = bx == 0 ? acc_s : 0.0;  // series pulls lie in the x = 0 / 1 cells
+    }
+}
+
+// Adds the per-block sums of one reverse step in block order (one block of 256 threads).
+__global__ void __launch_bounds__(256) k_param_sum(const double *__restrict__ partials, int nblocks,
+                                                   double *__restrict__ iso_acc, double *__restrict__ series,
+                                                   i64 step) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = int(threadIdx.x); i < nblocks; i += int(blockDim.x)) {
+        a += partials[3 * i64(i)];
+        b += partials[3 * i64(i) + 1];
+        c += partials[3 * i64(i) + 2];
+    }
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) {
+        if (iso_acc) {
+            iso_acc[0] += a;
+            iso_acc[1] += b;
         }
-        if (want_series) unsafeAtomicAdd(p.grad_series + step, acc_s);
+        if (series) series[step] += c;
     }
 }
 
 __global__ void __launch_bounds__(64) k_iso_final(const double *__restrict__ acc, double *__restrict__ grad) {
-    const int t = int(threadIdx.x);
-    double a = 0.0, b = 0.0;
-    if (t < FMX_ADJ_SLOTS) {
-        a = acc[t * FMX_ADJ_STRIDE];
-        b = acc[t * FMX_ADJ_STRIDE + 1];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-    }
-    if (t == 0) {
-        grad[0] = a;
-        grad[1] = b;
+    if (threadIdx.x == 0) {
+        grad[0] = acc[0];
+        grad[1] = acc[1];
     }
 }
 
@@ -479,6 +488,19 @@ __global__ void __launch_bounds__(256) k_ckpt_decode(const typename Storage<CP>:
 
 // ------------------------------------------------------------- host side
 inline i64 align256(i64 b) { return (b + 255) & ~i64(255); }
+
+// Device scratch owned for the duration of one call (freed after the stream's work on it).
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    int alloc(i64 bytes, const char *who) { return check_hip(hipMalloc(&p, size_t(bytes)), who); }
+    ~Scratch() {
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    }
+};
 
 // One stored state: f (19 planes), g (7 planes), q, phi, each section 256-byte aligned.
 template <int P> struct StateView {
@@ -584,9 +606,10 @@ static int step_between(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
 template <int P>
 static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, const StateView<P> &st,
                         const fmx_adjoint_bufs &adj, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
-                        i64 widx, int lam_cur, hipStream_t s) {
+                        i64 widx, int lam_cur, double *partials, hipStream_t s) {
     using C = typename Storage<P>::C;
     AdjointPtrs<P> ap{};
+    ap.partials = partials;
     ap.f = st.f;
     ap.g = st.g;
     ap.q = st.q;
@@ -603,7 +626,6 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
     ap.lam_phi = static_cast<C *>(adj.lam_phi);
     ap.grad_field = adj.grad_field;
     ap.grad_series = adj.grad_series;
-    ap.iso_acc = adj.iso_acc;
     ap.outlet_weight = adj.outlet_weight;
     const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
     const double inv_count = 1.0 / double(i64(d.ny) * d.nz);
@@ -618,7 +640,12 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
             hipLaunchKernelGGL((k_adjoint<P, FMX_NS_PBB>), grid, block, 0, s, ap, prm, step, widx, inv_count);
             break;
     }
-    return after_launch("k_adjoint");
+    if (int rc = after_launch("k_adjoint")) return rc;
+    double *series = d.ad_inlet == 2 ? adj.grad_series : nullptr;
+    if (!d.isotherm && !series) return FMX_OK;
+    hipLaunchKernelGGL(k_param_sum, dim3(1), dim3(256), 0, s, partials, int(grid.x), d.isotherm ? adj.iso_acc : nullptr,
+                       series, step);
+    return after_launch("k_param_sum");
 }
 
 // Checkpoints at the storage precision, or 16-bit checkpoints of an FP32 run.
@@ -692,6 +719,12 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
     for (const Z &zz : zeros)
         if (zz.p && zz.bytes)
             if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
+    // per-block parameter-gradient sums of one reverse step
+    Scratch scratch;
+    scratch.s = s;
+    if (int rc = scratch.alloc(3 * coupled_blocks(prm.gx, prm.gy, prm.gz) * i64(sizeof(double)), "fmx_adjoint_backward"))
+        return rc;
+    double *partials = static_cast<double *>(scratch.p);
     const bool recast = cp != P;  // 16-bit checkpoints of an FP32 run: decoded into seg slot 0
     const int nseg = (nsteps + k - 1) / k;
     int lam = 0;
@@ -730,7 +763,8 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
             for (int r = 1; r < mi; ++r)
                 if (int rc = step_between<P>(d, b, state(i, r - 1), state(i, r), prm, t0 + r - 1, s)) return rc;
             for (int r = mi - 1; r >= 0; --r) {
-                if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t0 + r, t0 + r - step0, lam, s)) return rc;
+                if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t0 + r, t0 + r - step0, lam, partials, s))
+                    return rc;
                 lam ^= 1;
             }
         }
