@@ -52,12 +52,13 @@ template <int P> struct AdjointPtrs {
     double *grad_field;
     double *grad_series;
     const double *outlet_weight;
-    double *partials;            // 3 per block: the block's parameter-gradient sums of this step
+    double *iso_part;            // 2 per block: the block's running dL/d iso_a, dL/d iso_b
+    double *ser_row;             // 1 per block: the block's inlet-series sum of this step (or null)
 };
 
 // Block sum of three values; lane 0 of the block receives the totals.
 __device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
-    __shared__ double red[3][4];
+    __shared__ double red[3][16];
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
         b += __shfl_xor(b, off, 64);
@@ -81,84 +82,105 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
     }
 }
 
-template <int P, int OP>
+// Addressing as in k_coupled: byte offsets are 32-bit (a scalar plane base plus a vector offset)
+// unless WIDE; rows are whole waves, so the y / z neighbour deltas are wave-uniform.  The collided
+// populations are recomputed where they are needed rather than held (fewer live registers, more
+// waves in flight); every sum runs in the same order as before, so the results do not depend on it.
+template <int P, int OP, bool WIDE>
 __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 widx, double inv_count) {
     using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    using Off = std::conditional_t<WIDE, i64, unsigned>;
+    constexpr unsigned SB = sizeof(S);
+    constexpr unsigned CB = sizeof(C);
     int bx, by, bz;
     block_coords(prm, bx, by, bz);
     const int x = bx * prm.block_x + int(threadIdx.x);
-    const int y = by * prm.block_y + int(threadIdx.y);
+    // block_x is a multiple of 64, so every wave lies in one row and y is wave-uniform
+    const int y = __builtin_amdgcn_readfirstlane(by * prm.block_y + int(threadIdx.y));
     const int z = bz;
     const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
     const i64 n = prm.n;
     double acc_a = 0.0, acc_b = 0.0, acc_s = 0.0;
 
     if (x < nx && y < ny) {
-        const unsigned cell = unsigned(x) + unsigned(nx) * (unsigned(y) + unsigned(ny) * unsigned(z));
-        const std::uint32_t lk = p.links[cell];
+        const Off cell = Off(x) + Off(nx) * (Off(y) + Off(ny) * Off(z));
+        const Off cell_c = cell * CB;
+        const Off plane_s = Off(n) * SB, plane_c = Off(n) * CB;
+        const std::uint32_t lk = ld_b(p.links, cell * 4u);
         const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
 
-        int ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
-        bool ym_ok = ym >= 0, yp_ok = yp < ny, zm_ok = zm >= 0, zp_ok = zp < nz;
+        // rows y - cy and z - cz (index cy + 1, cz + 1) relative to this one, and whether they exist
+        int dy[3] = {1, 0, -1}, dz[3] = {1, 0, -1};
+        bool yin[3] = {y + 1 < ny, true, y > 0}, zin[3] = {z + 1 < nz, true, z > 0};
         if (prm.py) {
-            if (!ym_ok) ym += ny;
-            if (!yp_ok) yp -= ny;
-            ym_ok = yp_ok = true;
+            if (!yin[0]) dy[0] = 1 - ny;
+            if (!yin[2]) dy[2] = ny - 1;
+            yin[0] = yin[2] = true;
         }
         if (prm.pz) {
-            if (!zm_ok) zm += nz;
-            if (!zp_ok) zp -= nz;
-            zm_ok = zp_ok = true;
+            if (!zin[0]) dz[0] = 1 - nz;
+            if (!zin[2]) dz[2] = nz - 1;
+            zin[0] = zin[2] = true;
         }
-        const unsigned ys[3] = {unsigned(ym), unsigned(y), unsigned(yp)};
-        const unsigned zs[3] = {unsigned(zm), unsigned(z), unsigned(zp)};
-        const bool yok[3] = {ym_ok, true, yp_ok};
-        const bool zok[3] = {zm_ok, true, zp_ok};
-        // cell (xx, y + dy, z + dz) in the box (after periodic wrap), and its index
-        auto inside = [&](int xx, int dy, int dz) -> bool { return xx >= 0 && xx < nx && yok[1 + dy] && zok[1 + dz]; };
-        auto at = [&](int xx, int dy, int dz) -> unsigned {
-            return unsigned(xx) + unsigned(nx) * (ys[1 + dy] + unsigned(ny) * zs[1 + dz]);
+        // does the cell (ox - cx, y - cy, z - cz) exist, and its index relative to the cell ox
+        auto rel_ok = [&](int ox, int cx, int cy, int cz) -> bool {
+            return ox - cx >= 0 && ox - cx < nx && yin[cy + 1] && zin[cz + 1];
         };
+        auto rel = [&](int cx, int cy, int cz) -> Off {
+            return Off(dy[cy + 1]) * Off(nx) + Off(dz[cz + 1]) * (Off(nx) * Off(ny)) - Off(cx);
+        };
+        // source of this cell's pull along c: the cell - c, or the cell itself outside the box
+        auto src = [&](int cx, int cy, int cz) -> Off { return rel_ok(x, cx, cy, cz) ? cell + rel(cx, cy, cz) : cell; };
 
         // ------------------------------------------------ forward recompute: pulls
         C fo[19];
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
-            bool bb = false;
-            if constexpr (OP == FMX_NS_BGK && q != 0) bb = (lk >> q) & 1u;
-            const auto raw = bb ? p.f[i64(Q19::opp[q]) * n + cell]
-                                : p.f[i64(q) * n + (inside(x - cx, -cy, -cz) ? at(x - cx, -cy, -cz) : cell)];
-            fo[q] = decode<P>(raw, w19<C>(q), prm.f_ref);
+            constexpr int o = Q19::opp[q];
+            constexpr bool BB = OP == FMX_NS_BGK && q != 0;
+            constexpr int bq = BB ? (q < o ? q : o) : q;
+            Off off = src(cx, cy, cz) * SB + Off(q - bq) * plane_s;
+            if constexpr (BB) {  // the bounce-back pull: this cell along opp(q)
+                const Off boff = cell * SB + Off(o - bq) * plane_s;
+                const Off m = Off(int(lk << (31 - q)) >> 31);
+                off = (boff & m) | (off & ~m);
+            }
+            fo[q] = decode<P>(ld_b(plane(p.f, bq, n), off), w19<C>(q), prm.f_ref);
         });
+        // (every load below is unconditional, from an address in the box, and selected afterwards:
+        // loads inside branches would each wait for their data before the next one is issued)
         const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
         C gi[7];
-        bool subst[7];
+        unsigned subst = 0;  // bit q: the pull along q reads the inlet series value
         static_for<0, 7>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-            const bool in = inside(x - cx, -cy, -cz);
+            const bool in = rel_ok(x, cx, cy, cz);
             const int sx = in ? x - cx : x;
-            subst[q] = prm.ad_inlet == 2 && sx == 0;
-            if (subst[q]) {
-                gi[q] = phi_pre * w7<C>(q);
+            const bool sub = prm.ad_inlet == 2 && sx == 0;
+            subst |= unsigned(sub) << q;
+            const Off s = (in ? cell + rel(cx, cy, cz) : cell) * SB;
+            const S raw = ld_b(plane(p.g, q, n), s);
+            C gq;
+            if constexpr (IsHalf<P>::value) {
+                // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
+                // on a Neumann outlet plane), as the forward step stores them
+                const Off sref = s * (CB / SB) - ((prm.ad_outlet && sx == nx - 1) ? Off(CB) : Off(0));
+                gq = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
             } else {
-                const unsigned s = in ? at(x - cx, -cy, -cz) : cell;
-                if constexpr (IsHalf<P>::value) {
-                    // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
-                    // on a Neumann outlet plane), as the forward step stores them
-                    const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                    gi[q] = decode<P>(p.g[i64(q) * n + s], w7<C>(q), p.phi[sref]);
-                } else {
-                    gi[q] = p.g[i64(q) * n + s];
-                }
+                gq = raw;
             }
+            gi[q] = sub ? phi_pre * w7<C>(q) : gq;
         });
+        C phis = C(0);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) phis += gi[q];
         const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
-        const C phi_lag = sorbs ? p.phi[cell] : C(0);
-        const C q_old = sorbs ? p.q[cell] : C(0);
-        const C fv = p.field ? p.field[cell] : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
+        const C phi_lag = sorbs ? ld_b(p.phi, cell_c) : C(0);
+        const C fv = p.field ? ld_b(p.field, cell_c) : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
         const C fcl = fv < C(0) ? C(0) : (fv > C(1) ? C(1) : fv);
         const bool f_pass = fv >= C(0) && fv <= C(1);
 
@@ -188,17 +210,17 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
         }
         const C u2x15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
         const C ns = (OP == FMX_NS_PBB) ? fcl : C(0);
-        C fc[19];  // collided populations (before the partial bounce-back blend)
-        static_for<0, 19>([&](auto qc) {
+        // collided population q (before the partial bounce-back blend)
+        auto fc = [&](auto qc) -> C {
             constexpr int q = decltype(qc)::value;
-            fc[q] = fo[q] + prm.omega * (feq_bgk<q>(rho, ux, uy, uz, u2x15) - fo[q]);
-        });
+            return fo[q] + prm.omega * (feq_bgk<q>(rho, ux, uy, uz, u2x15) - fo[q]);
+        };
         // post-collision moments of the stored result
         C r2 = C(0), vx = C(0), vy = C(0), vz = C(0);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C fr = fc[q];
-            if (OP == FMX_NS_PBB) fr = (C(1) - ns) * fc[q] + ns * fo[Q19::opp[q]];
+            C fr = fc(qc);
+            if (OP == FMX_NS_PBB) fr = (C(1) - ns) * fr + ns * fo[Q19::opp[q]];
             if (solid) fr = fo[q];
             r2 += fr;
             vx = axpy1<Q19::cx[q]>(vx, fr);
@@ -215,33 +237,52 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             ce = c - prm.min_conc;
             bc = prm.iso_b * ce;
         }
-        C phis = C(0);
-#pragma unroll
-        for (int q = 0; q < 7; ++q) phis += gi[q];
         const bool clamped = phis < prm.min_conc;
         const C phi = clamped ? prm.min_conc : phis;
-        const C om = p.tau_field ? C(1) / p.tau_field[cell] : prm.omega_ad;
+        const C om = p.tau_field ? C(1) / ld_b(p.tau_field, cell_c) : prm.omega_ad;
         const C ax = prm.coupling ? ux2 : C(0), ay = prm.coupling ? uy2 : C(0), az = prm.coupling ? uz2 : C(0);
 
         // ------------------------------------------------ gather the adjoint of what the step stored
-        // adjoint of slot (q, owner) of f: its reader's adjoint (see the header comment)
-        auto gather_f = [&](auto qc, int ox, unsigned oc, std::uint32_t lo) -> C {
+        // adjoint of slot (q, owner) of f, the owner at x = ox, index oc, link word lo: its readers'
+        // adjoint (the owner + c, or the owner itself through bounce-back; plus the owner's
+        // keep-local pull where the owner - c lies outside the box)
+        // The streaming / bounce-back reader's part (gf_main, gg_main; `on` false: none) is loaded
+        // unconditionally, the keep-local part only where the owner - c lies outside the box.
+        auto gf_main = [&](auto qc, int ox, Off oc, std::uint32_t lo, bool on) -> C {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q], oq = Q19::opp[q];
-            if (OP == FMX_NS_BGK && (lo & FMX_LINK_SOLID)) return p.lf_in[i64(oq) * n + oc];
-            C v = C(0);
-            if (OP == FMX_NS_BGK && q != 0 && ((lo >> oq) & 1u)) v = p.lf_in[i64(oq) * n + oc];
-            else if (inside(ox + cx, cy, cz)) v = p.lf_in[i64(q) * n + at(ox + cx, cy, cz)];
-            if (!inside(ox - cx, -cy, -cz)) v += p.lf_in[i64(q) * n + oc];
+            constexpr bool BB = OP == FMX_NS_BGK && q != 0;
+            constexpr int bq = BB ? (q < oq ? q : oq) : q;
+            const bool nb_ok = rel_ok(ox, -cx, -cy, -cz);
+            Off off = (nb_ok ? oc + rel(-cx, -cy, -cz) : oc) * CB + Off(q - bq) * plane_c;
+            bool take = nb_ok;
+            if constexpr (OP == FMX_NS_BGK) {
+                // a solid owner's slot q is read back by itself along opp(q); so is a bounce-back link's
+                const bool self = (lo & FMX_LINK_SOLID) || (BB && ((lo >> oq) & 1u));
+                off = self ? oc * CB + Off(oq - bq) * plane_c : off;
+                take = take || self;
+            }
+            const C v = ld_b(plane(p.lf_in, bq, n), off);
+            return (on && take) ? v : C(0);
+        };
+        auto gf_local = [&](auto qc, int ox, Off oc, std::uint32_t lo, bool on, C v) -> C {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
+            const bool o_solid = OP == FMX_NS_BGK && (lo & FMX_LINK_SOLID);
+            if (on && !o_solid && !rel_ok(ox, cx, cy, cz)) v += ld_b(plane(p.lf_in, q, n), oc * CB);
             return v;
         };
-        auto gather_g = [&](auto qc, int ox, unsigned oc) -> C {
+        auto gg_main = [&](auto qc, int ox, Off oc, bool on) -> C {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
-            if (prm.ad_inlet == 2 && ox == 0) return C(0);  // every reader takes the series value instead
-            C v = C(0);
-            if (inside(ox + cx, cy, cz)) v = p.lg_in[i64(q) * n + at(ox + cx, cy, cz)];
-            if (!inside(ox - cx, -cy, -cz)) v += p.lg_in[i64(q) * n + oc];
+            const bool nb_ok = rel_ok(ox, -cx, -cy, -cz);
+            const C v = ld_b(plane(p.lg_in, q, n), (nb_ok ? oc + rel(-cx, -cy, -cz) : oc) * CB);
+            return (on && nb_ok) ? v : C(0);
+        };
+        auto gg_local = [&](auto qc, int ox, Off oc, bool on, C v) -> C {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Q7::cx[q], cy = Q7::cy[q], cz = Q7::cz[q];
+            if (on && !rel_ok(ox, cx, cy, cz)) v += ld_b(plane(p.lg_in, q, n), oc * CB);
             return v;
         };
 
@@ -249,24 +290,39 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
         const bool g_const = (prm.ad_inlet == 1 && x == 0) || (prm.ad_outlet && x == nx - 1);
         const bool writes_outlet_f = prm.ns_outlet && x == nx - 2;
         const bool writes_outlet_g = prm.ad_outlet && x == nx - 2;
-        C lfr[19];
+        // slots of g on x = 0 under a series inlet: every reader takes the series value instead
+        const bool g_gather = !g_const && !(prm.ad_inlet == 2 && x == 0);
+        C lfr[19], lgo[7];
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            lfr[q] = f_const ? C(0) : gather_f(qc, x, cell, lk);
+            lfr[q] = gf_main(qc, x, cell, lk, !f_const);
         });
-        C lgo[7];
         static_for<0, 7>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            lgo[q] = g_const ? C(0) : gather_g(qc, x, cell);
-            if (writes_outlet_g) lgo[q] += gather_g(qc, x + 1, cell + 1u);
+            lgo[q] = gg_main(qc, x, cell, g_gather);
         });
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lfr[q] = gf_local(qc, x, cell, lk, !f_const, lfr[q]);
+        });
+        static_for<0, 7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lgo[q] = gg_local(qc, x, cell, g_gather, lgo[q]);
+        });
+        if (writes_outlet_g) {  // the Neumann outlet's copy of this cell's g (x + 1 = nx - 1 is never 0)
+            static_for<0, 7>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lgo[q] += gg_local(qc, x + 1, cell + 1u, true, gg_main(qc, x + 1, cell + 1u, true));
+            });
+        }
         C lux = C(0), luy = C(0), luz = C(0);  // adjoint of the post-collision velocity
         if (writes_outlet_f) {  // the outlet cell's populations feq(rho_out, u2) (PressureDirichlet.cu:81-95)
-            const std::uint32_t lo = p.links[cell + 1u];
+            const std::uint32_t lo = ld_b(p.links, (cell + 1u) * 4u);
             C sw = C(0);
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const C lp = gather_f(qc, x + 1, cell + 1u, lo) * w19<C>(q) * prm.rho_out;
+                const C lp = gf_local(qc, x + 1, cell + 1u, lo, true, gf_main(qc, x + 1, cell + 1u, lo, true)) *
+                             w19<C>(q) * prm.rho_out;
                 const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux2, uy2, uz2);
                 const C t = lp * C(3) * (C(1) + C(3) * eu);
                 lux = axpy1<Q19::cx[q]>(lux, t);
@@ -278,9 +334,9 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             luy -= C(3) * uy2 * sw;
             luz -= C(3) * uz2 * sw;
         }
-        const C lq_next = p.lam_q[cell];
+        const C lq_next = ld_b(p.lam_q, cell_c);
         const C seed = (p.outlet_weight && x == nx - 1) ? C(p.outlet_weight[widx] * inv_count) : C(0);
-        C lphi = p.lam_phi[cell] + seed;
+        C lphi = ld_b(p.lam_phi, cell_c) + seed;
 
         // ------------------------------------------------ AD collision: go = gi + om (geq - gi) + w S
         C lS = C(0), lax = C(0), lay = C(0), laz = C(0);
@@ -305,8 +361,8 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
         C ser = C(0);
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
-            if (subst[q]) ser += w7<C>(q) * lgi[q];
-            p.lg_out[i64(q) * n + cell] = lgi[q];
+            if ((subst >> q) & 1u) ser += w7<C>(q) * lgi[q];
+            st_plane_nt(plane(p.lg_out, q, n), cell_c, lgi[q]);
         }
         acc_s = double(ser);
 
@@ -325,11 +381,11 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                 acc_a = double(lqeq * ce);
                 lce = lqeq * prm.iso_a;
             }
-            p.lam_q[cell] = lq_next - prm.k_ldf * ldq;
-            p.lam_phi[cell] = phi_lag < prm.min_conc ? C(0) : lce;
+            st_b(p.lam_q, cell_c, C(lq_next - prm.k_ldf * ldq));
+            st_b(p.lam_phi, cell_c, phi_lag < prm.min_conc ? C(0) : lce);
         } else {
-            p.lam_q[cell] = C(0);
-            p.lam_phi[cell] = C(0);
+            st_b(p.lam_q, cell_c, C(0));
+            st_b(p.lam_phi, cell_c, C(0));
         }
 
         // ------------------------------------------------ NS: post-collision velocity u2 = v / r2
@@ -348,31 +404,24 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
         }
 
         // ------------------------------------------------ NS collision and pull
-        C lfo[19];
         if (solid) {
-#pragma unroll
-            for (int q = 0; q < 19; ++q) lfo[q] = lfr[q];
-        } else {
-            C lfield = C(0);
-            C lfc[19];
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                lfc[q] = lfr[q];
-                lfo[q] = C(0);
+                st_plane_nt(plane(p.lf_out, q, n), cell_c, lfr[q]);
             });
-            if (OP == FMX_NS_PBB) {  // fr = (1 - ns) fc + ns fo[opp]
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    lfield += lfr[q] * (fo[Q19::opp[q]] - fc[q]);
-                    lfo[Q19::opp[q]] += ns * lfr[q];
-                    lfc[q] = (C(1) - ns) * lfr[q];
-                });
-            }
-            // fc = fo + omega (feq(rho, u) - fo)
+        } else {
+            // fr = (1 - ns) fc + ns fo[opp] (partial bounce-back), fc = fo + omega (feq(rho, u) - fo):
+            // first the sums over q, then each pulled population's adjoint
+            C lfield = C(0);
             C lrho = C(0), lvx = C(0), lvy = C(0), lvz = C(0), sw = C(0);
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const C lfeq = prm.omega * lfc[q];
+                C lfc = lfr[q];
+                if (OP == FMX_NS_PBB) {
+                    lfield += lfr[q] * (fo[Q19::opp[q]] - fc(qc));
+                    lfc = (C(1) - ns) * lfr[q];
+                }
+                const C lfeq = prm.omega * lfc;
                 const C w = w19<C>(q);
                 const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
                 const C cu = C(3) * eu;
@@ -382,7 +431,6 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                 lvy = axpy1<Q19::cy[q]>(lvy, t);
                 lvz = axpy1<Q19::cz[q]>(lvz, t);
                 sw += lfeq * w;
-                lfo[q] += (C(1) - prm.omega) * lfc[q];
             });
             C ljx = C(0), ljy = C(0), ljz = C(0);
             if (rho_ok) {  // the rho <= 0 fallback uses constants
@@ -401,52 +449,60 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             }
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                lfo[q] += lrho + dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ljx, ljy, ljz);
+                C lfo = C(0);
+                if (OP == FMX_NS_PBB) lfo += ns * lfr[Q19::opp[q]];
+                const C lfc = (OP == FMX_NS_PBB) ? (C(1) - ns) * lfr[q] : lfr[q];
+                lfo += (C(1) - prm.omega) * lfc;
+                lfo += lrho + dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ljx, ljy, ljz);
+                st_plane_nt(plane(p.lf_out, q, n), cell_c, lfo);
             });
-            if ((OP == FMX_NS_PBB || OP == FMX_NS_POROUS) && p.grad_field && p.field && f_pass)
-                p.grad_field[cell] += double(lfield);
+            if ((OP == FMX_NS_PBB || OP == FMX_NS_POROUS) && p.grad_field && p.field && f_pass) {
+                const Off gb = cell * 8u;
+                st_b(p.grad_field, gb, ld_b(p.grad_field, gb) + double(lfield));
+            }
         }
-#pragma unroll
-        for (int q = 0; q < 19; ++q) p.lf_out[i64(q) * n + cell] = lfo[q];
     }
 
     // ---------------------------------------------------- parameter gradients
-    // Every block stores its sums; k_param_sum then adds them in block order, so the gradients repeat
-    // to the bit from run to run.
-    if (!prm.isotherm && !(prm.ad_inlet == 2 && p.grad_series)) return;  // grid-uniform
+    // Block b covers the same cells in every step: it keeps its own running isotherm sums and stores
+    // its inlet-series sum of this step; k_series_rows and k_iso_final add them in block order.  No
+    // atomics, so the gradients repeat to the bit from run to run.
+    const bool series = prm.ad_inlet == 2 && p.ser_row;
+    if (!prm.isotherm && !series) return;  // grid-uniform
     block_sum3(acc_a, acc_b, acc_s);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        double *pp = p.partials + 3 * i64(blockIdx.x);
-        pp[0] = acc_a;
-        pp[1] = acc_b;
-        pp[2] = bx == 0 ? acc_s : 0.0;  // series pulls lie in the x = 0 / 1 cells
+        if (prm.isotherm) {
+            double *ip = p.iso_part + 2 * i64(blockIdx.x);
+            ip[0] += acc_a;
+            ip[1] += acc_b;
+        }
+        if (series) p.ser_row[blockIdx.x] = bx == 0 ? acc_s : 0.0;  // series pulls lie in the x = 0 / 1 cells
     }
 }
 
-// Adds the per-block sums of one reverse step in block order (one block of 256 threads).
-__global__ void __launch_bounds__(256) k_param_sum(const double *__restrict__ partials, int nblocks,
-                                                   double *__restrict__ iso_acc, double *__restrict__ series,
-                                                   i64 step) {
+// Row r of the series buffer holds the per-block sums of reverse step top - r; one workgroup per
+// row adds them in block order.
+__global__ void __launch_bounds__(256) k_series_rows(const double *__restrict__ rows, int nblocks,
+                                                     double *__restrict__ series, i64 top) {
+    const double *row = rows + i64(blockIdx.x) * nblocks;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = int(threadIdx.x); i < nblocks; i += int(blockDim.x)) c += row[i];
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) series[top - i64(blockIdx.x)] += c;
+}
+
+// The per-block isotherm sums added in block order.
+__global__ void __launch_bounds__(1024) k_iso_final(const double *__restrict__ part, int nblocks,
+                                                    double *__restrict__ grad) {
     double a = 0.0, b = 0.0, c = 0.0;
     for (int i = int(threadIdx.x); i < nblocks; i += int(blockDim.x)) {
-        a += partials[3 * i64(i)];
-        b += partials[3 * i64(i) + 1];
-        c += partials[3 * i64(i) + 2];
+        a += part[2 * i64(i)];
+        b += part[2 * i64(i) + 1];
     }
     block_sum3(a, b, c);
     if (threadIdx.x == 0) {
-        if (iso_acc) {
-            iso_acc[0] += a;
-            iso_acc[1] += b;
-        }
-        if (series) series[step] += c;
-    }
-}
-
-__global__ void __launch_bounds__(64) k_iso_final(const double *__restrict__ acc, double *__restrict__ grad) {
-    if (threadIdx.x == 0) {
-        grad[0] = acc[0];
-        grad[1] = acc[1];
+        grad[0] = a;
+        grad[1] = b;
     }
 }
 
@@ -606,10 +662,11 @@ static int step_between(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
 template <int P>
 static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, const StateView<P> &st,
                         const fmx_adjoint_bufs &adj, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
-                        i64 widx, int lam_cur, double *partials, hipStream_t s) {
+                        i64 widx, int lam_cur, double *iso_part, double *ser_row, hipStream_t s) {
     using C = typename Storage<P>::C;
     AdjointPtrs<P> ap{};
-    ap.partials = partials;
+    ap.iso_part = iso_part;
+    ap.ser_row = ser_row;
     ap.f = st.f;
     ap.g = st.g;
     ap.q = st.q;
@@ -629,23 +686,27 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
     ap.outlet_weight = adj.outlet_weight;
     const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
     const double inv_count = 1.0 / double(i64(d.ny) * d.nz);
+    // 64-bit offsets when a pair of FP64 planes could pass 4 GiB (FMX_WIDE=1 forces them)
+    const char *fw = std::getenv("FMX_WIDE");
+    const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(double)) > (i64(1) << 32);
+#define FMX_ADJ_LAUNCH(OPV, W) \
+    hipLaunchKernelGGL((k_adjoint<P, OPV, W>), grid, block, 0, s, ap, prm, step, widx, inv_count)
     switch (d.ns_op) {
         case FMX_NS_BGK:
-            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_BGK>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            if (wide) FMX_ADJ_LAUNCH(FMX_NS_BGK, true);
+            else FMX_ADJ_LAUNCH(FMX_NS_BGK, false);
             break;
         case FMX_NS_POROUS:
-            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_POROUS>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            if (wide) FMX_ADJ_LAUNCH(FMX_NS_POROUS, true);
+            else FMX_ADJ_LAUNCH(FMX_NS_POROUS, false);
             break;
         default:
-            hipLaunchKernelGGL((k_adjoint<P, FMX_NS_PBB>), grid, block, 0, s, ap, prm, step, widx, inv_count);
+            if (wide) FMX_ADJ_LAUNCH(FMX_NS_PBB, true);
+            else FMX_ADJ_LAUNCH(FMX_NS_PBB, false);
             break;
     }
-    if (int rc = after_launch("k_adjoint")) return rc;
-    double *series = d.ad_inlet == 2 ? adj.grad_series : nullptr;
-    if (!d.isotherm && !series) return FMX_OK;
-    hipLaunchKernelGGL(k_param_sum, dim3(1), dim3(256), 0, s, partials, int(grid.x), d.isotherm ? adj.iso_acc : nullptr,
-                       series, step);
-    return after_launch("k_param_sum");
+#undef FMX_ADJ_LAUNCH
+    return after_launch("k_adjoint");
 }
 
 // Checkpoints at the storage precision, or 16-bit checkpoints of an FP32 run.
@@ -713,18 +774,33 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
                  {adj.lam_g[0], size_t(7 * n) * sizeof(C)},
                  {adj.lam_q, size_t(n) * sizeof(C)},
                  {adj.lam_phi, size_t(n) * sizeof(C)},
-                 {adj.iso_acc, size_t(FMX_ADJ_SLOTS * FMX_ADJ_STRIDE) * sizeof(double)},
                  {adj.grad_field, adj.grad_field ? size_t(n) * sizeof(double) : 0},
                  {adj.grad_series ? adj.grad_series + step0 : nullptr, size_t(nsteps) * sizeof(double)}};
     for (const Z &zz : zeros)
         if (zz.p && zz.bytes)
             if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
-    // per-block parameter-gradient sums of one reverse step
+    // per-block isotherm sums, and the per-block series sums of up to kRows reverse steps
+    constexpr int kRows = 64;
+    const int nblocks = int(coupled_blocks(prm.gx, prm.gy, prm.gz));
+    const bool want_series = d.ad_inlet == 2 && adj.grad_series;
     Scratch scratch;
     scratch.s = s;
-    if (int rc = scratch.alloc(3 * coupled_blocks(prm.gx, prm.gy, prm.gz) * i64(sizeof(double)), "fmx_adjoint_backward"))
+    if (int rc = scratch.alloc((2 + (want_series ? kRows : 0)) * i64(nblocks) * i64(sizeof(double)),
+                               "fmx_adjoint_backward"))
         return rc;
-    double *partials = static_cast<double *>(scratch.p);
+    double *iso_part = static_cast<double *>(scratch.p);
+    double *ser_rows = iso_part + 2 * i64(nblocks);
+    if (int rc = check_hip(hipMemsetAsync(iso_part, 0, 2 * size_t(nblocks) * sizeof(double), s),
+                           "fmx_adjoint_backward: clear"))
+        return rc;
+    int rows = 0;  // rows filled; row r holds reverse step top - r
+    i64 top = 0;
+    auto flush_rows = [&]() -> int {
+        if (rows == 0) return FMX_OK;
+        hipLaunchKernelGGL(k_series_rows, dim3(rows), dim3(256), 0, s, ser_rows, nblocks, adj.grad_series, top);
+        rows = 0;
+        return after_launch("k_series_rows");
+    };
     const bool recast = cp != P;  // 16-bit checkpoints of an FP32 run: decoded into seg slot 0
     const int nseg = (nsteps + k - 1) / k;
     int lam = 0;
@@ -763,13 +839,23 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
             for (int r = 1; r < mi; ++r)
                 if (int rc = step_between<P>(d, b, state(i, r - 1), state(i, r), prm, t0 + r - 1, s)) return rc;
             for (int r = mi - 1; r >= 0; --r) {
-                if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t0 + r, t0 + r - step0, lam, partials, s))
-                    return rc;
+                const i64 t = t0 + r;
+                double *row = nullptr;
+                if (want_series) {
+                    if (rows > 0 && t != top - rows)
+                        if (int rc = flush_rows()) return rc;
+                    if (rows == 0) top = t;
+                    row = ser_rows + i64(rows) * nblocks;
+                }
+                if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t, t - step0, lam, iso_part, row, s)) return rc;
                 lam ^= 1;
+                if (want_series && ++rows == kRows)
+                    if (int rc = flush_rows()) return rc;
             }
         }
     }
-    hipLaunchKernelGGL(k_iso_final, dim3(1), dim3(64), 0, s, adj.iso_acc, adj.grad_iso);
+    if (int rc = flush_rows()) return rc;
+    hipLaunchKernelGGL(k_iso_final, dim3(1), dim3(1024), 0, s, iso_part, nblocks, adj.grad_iso);
     return after_launch("k_iso_final");
 }
 

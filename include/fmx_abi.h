@@ -242,7 +242,8 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  * about 2 sqrt(k)), the sweep keeps sub-checkpoints every m steps within a segment and recomputes
  * one sub-segment at a time (one more forward pass over each segment; the same gradients, bit for
  * bit).  All sums over the grid run in a fixed order: a call repeats its gradients to the bit.
- * fmx_adjoint_backward allocates 24 bytes per launch block of device scratch for the call. */
+ * fmx_adjoint_backward allocates device scratch for the call: 16 bytes per launch block (256
+ * cells), plus 512 bytes per block with an inlet-series gradient. */
 #define FMX_ADJ_SLOTS 64
 #define FMX_ADJ_STRIDE 16 /* doubles between accumulator slots */
 
@@ -250,7 +251,7 @@ typedef struct fmx_adjoint_bufs {
     void *lam_f[2];              /* 19*N each, compute precision: adjoint populations (scratch) */
     void *lam_g[2];              /* 7*N each (scratch) */
     void *lam_q, *lam_phi;       /* N each (scratch) */
-    double *iso_acc;             /* FMX_ADJ_SLOTS*FMX_ADJ_STRIDE doubles (scratch; [0], [1] used) */
+    double *iso_acc;             /* FMX_ADJ_SLOTS*FMX_ADJ_STRIDE doubles (reserved; not written) */
     const double *outlet_weight; /* nsteps: dL/dC_out(step0 + i); NULL = no loss (all gradients 0) */
     double *grad_iso;            /* 2 doubles (written) */
     double *grad_field;          /* N doubles (written) or NULL */
