@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
     ap.add_argument("--no-large-box", action="store_true", help="skip the 512^3 (C3) timing per storage precision")
+    ap.add_argument("--adjoint-steps", type=int, default=None, metavar="N",
+                    help="steps of the adjoint timing (default: the workload's own)")
     ap.add_argument("--adjoint-c5", action="store_true",
                     help="only the adjoint/forward timing on C5 as specified (256x128x128, 20000 steps, FP16 "
                          "checkpoints every 500 steps, FP32 recompute; one-level and two-level segment stores)")
@@ -248,12 +250,13 @@ def main():
     if args.adjoint_c5:
         # one level: 499 FP32 segment slots (about 220 GiB); two levels: about 2 sqrt(500) slots
         for two_level in (False, True):
-            print(json.dumps({"adjoint": adjoint_timing("C5", "f32", k=500, ckpt="f16", two_level=two_level)}),
-                  flush=True)
+            print(json.dumps({"adjoint": adjoint_timing("C5", "f32", k=500, steps=args.adjoint_steps, ckpt="f16",
+                                                        two_level=two_level)}), flush=True)
         return
 
     if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_timing(prec="f64" if args.prec == "f64" else "f32")}), flush=True)
+        print(json.dumps({"adjoint": adjoint_timing(prec="f64" if args.prec == "f64" else "f32",
+                                                    steps=args.adjoint_steps)}), flush=True)
         return
 
     if args.calibrate_copy:
