@@ -414,19 +414,25 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             // first the sums over q, then each pulled population's adjoint
             C lfield = C(0);
             C lrho = C(0), lvx = C(0), lvy = C(0), lvz = C(0), sw = C(0);
+            // The per-direction terms below are recomputed from copies of rho and u that the compiler
+            // cannot see through: otherwise it keeps the equilibria of the forward recompute above
+            // live (19 or more registers) all the way here instead of recomputing them.
+            C rr = rho, rux = ux, ruy = uy, ruz = uz, ru2 = u2x15;
+            asm volatile("" : "+v"(rr), "+v"(rux), "+v"(ruy), "+v"(ruz), "+v"(ru2));
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 C lfc = lfr[q];
                 if (OP == FMX_NS_PBB) {
-                    lfield += lfr[q] * (fo[Q19::opp[q]] - fc(qc));
+                    const C fo_q = fo[q];
+                    lfield += lfr[q] * (fo[Q19::opp[q]] - (fo_q + prm.omega * (feq_bgk<q>(rr, rux, ruy, ruz, ru2) - fo_q)));
                     lfc = (C(1) - ns) * lfr[q];
                 }
                 const C lfeq = prm.omega * lfc;
                 const C w = w19<C>(q);
-                const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);
+                const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(rux, ruy, ruz);
                 const C cu = C(3) * eu;
-                lrho += lfeq * w * (C(1) + cu + C(0.5) * cu * cu - u2x15);
-                const C t = lfeq * w * rho * C(3) * (C(1) + cu);
+                lrho += lfeq * w * (C(1) + cu + C(0.5) * cu * cu - ru2);
+                const C t = lfeq * w * rr * C(3) * (C(1) + cu);
                 lvx = axpy1<Q19::cx[q]>(lvx, t);
                 lvy = axpy1<Q19::cy[q]>(lvy, t);
                 lvz = axpy1<Q19::cz[q]>(lvz, t);
@@ -447,11 +453,13 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             } else {
                 lrho = C(0);
             }
+            C rns = ns;  // (recomputed, not carried over from the sums above; see rr)
+            asm volatile("" : "+v"(rns));
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 C lfo = C(0);
-                if (OP == FMX_NS_PBB) lfo += ns * lfr[Q19::opp[q]];
-                const C lfc = (OP == FMX_NS_PBB) ? (C(1) - ns) * lfr[q] : lfr[q];
+                if (OP == FMX_NS_PBB) lfo += rns * lfr[Q19::opp[q]];
+                const C lfc = (OP == FMX_NS_PBB) ? (C(1) - rns) * lfr[q] : lfr[q];
                 lfo += (C(1) - prm.omega) * lfc;
                 lfo += lrho + dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ljx, ljy, ljz);
                 st_plane_nt(plane(p.lf_out, q, n), cell_c, lfo);
