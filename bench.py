@@ -234,8 +234,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the slab-decomposed multi-GPU path is a design "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only (the slab-decomposed multi-GPU path is a design, "
+                                   "not built), so --gpus must be 1"}))
         sys.exit(2)
 
     import torch
