@@ -324,6 +324,26 @@ def test_gradients_repeat_bitwise(pbb_problem, prec):
         assert np.array_equal(g1[name], g2[name]), name
 
 
+@pytest.mark.parametrize("env", [{"FMX_WIDE": "1"}, {"FMX_BLOCK": "128x2"}, {"FMX_BLOCK": "64x1", "FMX_XCD_REMAP": "0"}])
+def test_adjoint_launch_variants(pbb_problem, monkeypatch, env):
+    """The adjoint kernel with 64-bit offsets and under other launch shapes: the per-cell gradients
+    agree to the bit; the sums over the grid group cells by launch block, so the isotherm and
+    inlet-series gradients agree to 1e-12 (to the bit with 64-bit offsets, whose blocks are the same)."""
+    g = pbb_problem.grads()
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    say(f"  adjoint launch variant {env}")
+    g2 = Problem(adj_config("pbb"), target=pbb_problem.target).grads()
+    assert np.array_equal(g["field"], g2["field"])
+    if "FMX_WIDE" in env:
+        assert g["iso_a"] == g2["iso_a"] and g["iso_b"] == g2["iso_b"]
+        assert np.array_equal(g["series"], g2["series"])
+    else:
+        for name in ("iso_a", "iso_b"):
+            assert rel(g2[name], g[name]) < 1e-12, name
+        assert np.max(np.abs(g2["series"] - g["series"])) <= 1e-12 * np.max(np.abs(g["series"]))
+
+
 def test_c4_recovers_isotherm_constants():
     """The C4 inverse problem as specified: the target curve is the forward run with q_max x 1.1 and
     K_L x 0.9, so the minimum of L = sum_s (C_out(s) - C*(s))^2 is known (0, at those constants).
