@@ -21,7 +21,6 @@ NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE = 64, 16
-ADJ_SLOTS, ADJ_STRIDE = 64, 16
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
 
@@ -64,7 +63,7 @@ class AdjointBufs(C.Structure):
     _fields_ = [
         ("lam_f", C.c_void_p * 2), ("lam_g", C.c_void_p * 2),
         ("lam_q", C.c_void_p), ("lam_phi", C.c_void_p),
-        ("iso_acc", C.c_void_p), ("outlet_weight", C.c_void_p),
+        ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("outlet_weight", C.c_void_p),
         ("grad_iso", C.c_void_p), ("grad_field", C.c_void_p), ("grad_series", C.c_void_p),
     ]
 
@@ -122,6 +121,7 @@ _SIGS = {
     "fmx_adjoint_ckpt_bytes": ([_DP, _i], C.c_int64),
     "fmx_adjoint_forward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _i, _vp, _i, _vp], _i),
     "fmx_adjoint_seg_slots": ([C.c_int32, _i, _i], C.c_int64),
+    "fmx_adjoint_workspace_bytes": ([_DP], C.c_int64),
     "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _i, _vp, _i64, C.POINTER(AdjointBufs), _vp],
                              _i),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),

@@ -72,7 +72,7 @@ class AdjointRun:
         self.lam_g = [torch.empty(7 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
         self.lam_q = torch.empty(n, dtype=self.cdt, device=self.device)
         self.lam_phi = torch.empty(n, dtype=self.cdt, device=self.device)
-        self.iso_acc = torch.empty(abi.ADJ_SLOTS * abi.ADJ_STRIDE, dtype=torch.float64, device=self.device)
+        self.workspace = None  # sized at backward() (it follows the launch shape)
         self.grad_iso = torch.zeros(2, dtype=torch.float64, device=self.device)
         self.grad_field = torch.zeros(n, dtype=torch.float64, device=self.device) if self.sim.field is not None else None
         self.grad_series = (torch.zeros(self.sim.inlet_series.numel(), dtype=torch.float64, device=self.device)
@@ -130,12 +130,17 @@ class AdjointRun:
         a.lam_f[0], a.lam_f[1] = self.lam_f[0].data_ptr(), self.lam_f[1].data_ptr()
         a.lam_g[0], a.lam_g[1] = self.lam_g[0].data_ptr(), self.lam_g[1].data_ptr()
         a.lam_q, a.lam_phi = self.lam_q.data_ptr(), self.lam_phi.data_ptr()
-        a.iso_acc = self.iso_acc.data_ptr()
+        sim = self.sim
+        wb = int(abi.lib().fmx_adjoint_workspace_bytes(sim._desc_p()))
+        if wb < 0:
+            raise abi.FmxError("fmx_adjoint_workspace_bytes: bad descriptor")
+        if self.workspace is None or self.workspace.numel() < wb:
+            self.workspace = torch.empty(max(wb, 1), dtype=torch.uint8, device=self.device)
+        a.workspace, a.workspace_bytes = self.workspace.data_ptr(), self.workspace.numel()
         a.outlet_weight = w.data_ptr()
         a.grad_iso = self.grad_iso.data_ptr()
         a.grad_field = self.grad_field.data_ptr() if self.grad_field is not None else None
         a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
-        sim = self.sim
         abi.call("fmx_adjoint_backward", sim._desc_p(), sim._bufs_p(), 1, self.nsteps, self.k, self.ckpt.data_ptr(),
                  self.ckpt_prec, self.seg.data_ptr(), self.seg_slots, abi.C.byref(a), sim._s())
         sim.synchronize()

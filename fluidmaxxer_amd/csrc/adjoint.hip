@@ -553,18 +553,16 @@ __global__ void __launch_bounds__(256) k_ckpt_decode(const typename Storage<CP>:
 // ------------------------------------------------------------- host side
 inline i64 align256(i64 b) { return (b + 255) & ~i64(255); }
 
-// Device scratch owned for the duration of one call (freed after the stream's work on it).
-struct Scratch {
-    void *p = nullptr;
-    hipStream_t s = nullptr;
-    int alloc(i64 bytes, const char *who) { return check_hip(hipMalloc(&p, size_t(bytes)), who); }
-    ~Scratch() {
-        if (p) {
-            (void)hipStreamSynchronize(s);
-            (void)hipFree(p);
-        }
-    }
-};
+// Reverse steps whose per-block inlet-series sums the workspace holds before they are added up.
+constexpr int kSeriesRows = 64;
+
+// Workspace of a reverse sweep: 2 running isotherm sums per launch block, and with a per-step inlet
+// series kSeriesRows rows of one sum per block.
+inline i64 workspace_bytes(const fmx_coupled_desc &d) {
+    const CoupledParams<float> prm = make_params<float>(d);
+    const i64 nblocks = coupled_blocks(prm.gx, prm.gy, prm.gz);
+    return (2 + (d.ad_inlet == 2 ? kSeriesRows : 0)) * nblocks * i64(sizeof(double));
+}
 
 // One stored state: f (19 planes), g (7 planes), q, phi, each section 256-byte aligned.
 template <int P> struct StateView {
@@ -787,16 +785,11 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
     for (const Z &zz : zeros)
         if (zz.p && zz.bytes)
             if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
-    // per-block isotherm sums, and the per-block series sums of up to kRows reverse steps
-    constexpr int kRows = 64;
+    // the workspace: per-block isotherm sums, then the per-block series sums of up to kSeriesRows
+    // reverse steps
     const int nblocks = int(coupled_blocks(prm.gx, prm.gy, prm.gz));
     const bool want_series = d.ad_inlet == 2 && adj.grad_series;
-    Scratch scratch;
-    scratch.s = s;
-    if (int rc = scratch.alloc((2 + (want_series ? kRows : 0)) * i64(nblocks) * i64(sizeof(double)),
-                               "fmx_adjoint_backward"))
-        return rc;
-    double *iso_part = static_cast<double *>(scratch.p);
+    double *iso_part = static_cast<double *>(adj.workspace);
     double *ser_rows = iso_part + 2 * i64(nblocks);
     if (int rc = check_hip(hipMemsetAsync(iso_part, 0, 2 * size_t(nblocks) * sizeof(double), s),
                            "fmx_adjoint_backward: clear"))
@@ -857,7 +850,7 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
                 }
                 if (int rc = adjoint_step<P>(d, b, state(i, r), adj, prm, t, t - step0, lam, iso_part, row, s)) return rc;
                 lam ^= 1;
-                if (want_series && ++rows == kRows)
+                if (want_series && ++rows == kSeriesRows)
                     if (int rc = flush_rows()) return rc;
             }
         }
@@ -911,6 +904,11 @@ int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, in
     FMX_TRY_END
 }
 
+int64_t fmx_adjoint_workspace_bytes(const fmx_coupled_desc *d) {
+    if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1) return -1;
+    return workspace_bytes(*d);
+}
+
 int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level) {
     if (k < 1) return -1;
     return two_level ? seg_slots_min(k, recast != 0) : seg_slots_single(k, recast != 0);
@@ -930,8 +928,13 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
                          "least %lld segment slots, %lld for one level; %lld given)",
                          kk, (long long)need, (long long)seg_slots_single(kk, recast), (long long)seg_slots);
     if (!adj->lam_f[0] || !adj->lam_f[1] || !adj->lam_g[0] || !adj->lam_g[1] || !adj->lam_q || !adj->lam_phi ||
-        !adj->iso_acc || !adj->grad_iso)
+        !adj->grad_iso)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: missing adjoint buffer");
+    const i64 wb = workspace_bytes(*d);
+    if (!adj->workspace || adj->workspace_bytes < wb)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: workspace of %lld bytes given, %lld needed "
+                                      "(fmx_adjoint_workspace_bytes)",
+                         (long long)(adj->workspace ? adj->workspace_bytes : 0), (long long)wb);
     if (d->ad_inlet != 2 && adj->grad_series)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: grad_series needs the per-step inlet series (ad_inlet 2)");
     hipStream_t s = as_stream(stream);

@@ -241,17 +241,15 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  * Two levels: given fewer segment slots than that (at least fmx_adjoint_seg_slots(k, recast, 1),
  * about 2 sqrt(k)), the sweep keeps sub-checkpoints every m steps within a segment and recomputes
  * one sub-segment at a time (one more forward pass over each segment; the same gradients, bit for
- * bit).  All sums over the grid run in a fixed order: a call repeats its gradients to the bit.
- * fmx_adjoint_backward allocates device scratch for the call: 16 bytes per launch block (256
- * cells), plus 512 bytes per block with an inlet-series gradient. */
-#define FMX_ADJ_SLOTS 64
-#define FMX_ADJ_STRIDE 16 /* doubles between accumulator slots */
+ * bit).  All sums over the grid run in a fixed order (no atomics): a call repeats its gradients to
+ * the bit.  The sweep allocates nothing; its scratch is the caller's `workspace`. */
 
 typedef struct fmx_adjoint_bufs {
     void *lam_f[2];              /* 19*N each, compute precision: adjoint populations (scratch) */
     void *lam_g[2];              /* 7*N each (scratch) */
     void *lam_q, *lam_phi;       /* N each (scratch) */
-    double *iso_acc;             /* FMX_ADJ_SLOTS*FMX_ADJ_STRIDE doubles (reserved; not written) */
+    void *workspace;             /* fmx_adjoint_workspace_bytes(desc) bytes (scratch) */
+    int64_t workspace_bytes;     /* size of workspace */
     const double *outlet_weight; /* nsteps: dL/dC_out(step0 + i); NULL = no loss (all gradients 0) */
     double *grad_iso;            /* 2 doubles (written) */
     double *grad_field;          /* N doubles (written) or NULL */
@@ -269,6 +267,11 @@ int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *desc, int ckpt_prec);
  * update path). */
 int fmx_adjoint_forward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
                         int32_t k, int cur, void *ckpt, int ckpt_prec, void *stream);
+/* Device scratch bytes fmx_adjoint_backward needs (per-launch-block running sums; with a per-step
+ * inlet series also the per-block sums of 64 reverse steps): 16 bytes per 256-cell launch block,
+ * 528 with ad_inlet 2.  It follows the launch shape (FMX_BLOCK / FMX_TILE, as the kernels do).
+ * -1 for a bad descriptor. */
+int64_t fmx_adjoint_workspace_bytes(const fmx_coupled_desc *desc);
 /* Segment slots (of fmx_adjoint_state_bytes) a reverse sweep over segments of k steps needs:
  * one level (two_level = 0) or the two-level minimum (two_level = 1); recast = 1 for 16-bit
  * checkpoints of an FP32 run.  -1 for k < 1. */

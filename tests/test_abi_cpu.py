@@ -64,7 +64,9 @@ def test_adjoint_state_bytes_and_argument_checks():
     a = abi.AdjointBufs()
     for i in range(2):
         a.lam_f[i] = a.lam_g[i] = 0x1000
-    a.lam_q = a.lam_phi = a.iso_acc = a.grad_iso = 0x1000
+    a.lam_q = a.lam_phi = a.grad_iso = 0x1000
+    a.workspace = 0x1000
+    a.workspace_bytes = 1 << 40
 
     def backward(desc, k=5):
         return L.fmx_adjoint_backward(C.byref(desc), C.byref(b), 1, 10, k, 0x1000, desc.prec, 0x1000, k,
@@ -88,6 +90,16 @@ def test_adjoint_state_bytes_and_argument_checks():
     assert L.fmx_adjoint_seg_slots(64, 0, 1) == 14 and L.fmx_adjoint_seg_slots(64, 1, 1) == 15
     assert L.fmx_adjoint_seg_slots(500, 0, 1) == 43 and L.fmx_adjoint_seg_slots(1, 0, 1) == 0
     assert L.fmx_adjoint_seg_slots(0, 0, 0) == -1
+    # workspace: 2 doubles per launch block, 66 with a per-step inlet series; a short one is refused
+    ws = L.fmx_adjoint_workspace_bytes(C.byref(d))
+    assert ws > 0 and ws % 16 == 0
+    d.ad_inlet = 2
+    assert L.fmx_adjoint_workspace_bytes(C.byref(d)) == ws // 16 * (66 * 8)
+    d.ad_inlet = cfg.desc(abi.F32).ad_inlet
+    a.workspace_bytes = ws - 8
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 10, 5, 0x1000, abi.F32, 0x1000, 5, C.byref(a), None)
+    assert rc != 0 and b"workspace" in L.fmx_last_error()
+    a.workspace_bytes = 1 << 40
     rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 1, 200, 64, 0x1000, abi.F32, 0x1000, 13, C.byref(a), None)
     assert rc != 0 and b"at least 14 segment slots" in L.fmx_last_error()
     d.ns_inlet = 1
