@@ -47,25 +47,34 @@ def cpu_info():
 
 
 def cpu_baseline(cfg, prec, budget_s):
-    """The CPU oracle (serial restatement of the reference update) on a bounded sample of the workload."""
+    """The CPU oracle (restatement of the reference update) on a bounded sample of the workload: serial
+    (the baseline), then the same steps with its per-cell loops on all the threads OpenMP is given."""
     from oracle import ref_cpu as R
     from tests.lattice import oracle_setup
 
     oprec = 8 if prec == "f64" else 4
-    sysm = R.System(oracle_setup(cfg, R), oprec)
-    _, _, _, t1 = sysm.run(1)
-    steps = int(max(1, min(200, budget_s / max(t1, 1e-6))))
-    _, _, _, t = sysm.run(steps)
-    sysm.close()
     model, ncpu = cpu_info()
-    return {
-        "value": round(cfg.n * steps / t / 1e6, 3),
-        "unit": "MLUPS",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{cfg.meta['workload']} {cfg.nx}x{cfg.ny}x{cfg.nz}, {'FP64' if oprec == 8 else 'FP32'} serial oracle, "
-                  f"{steps} steps after 1 untimed step ({t:.1f} s), host '{model}' ({ncpu} logical CPUs)",
-    }
+    out = []
+    steps = None
+    for threads in (1, 0):  # 0: the OpenMP default (OMP_NUM_THREADS, else every logical CPU)
+        used = int(R.lib().orc_set_threads(threads))
+        sysm = R.System(oracle_setup(cfg, R), oprec)
+        _, _, _, t1 = sysm.run(1)
+        if steps is None:
+            steps = int(max(1, min(200, budget_s / max(t1, 1e-6))))
+        _, _, _, t = sysm.run(steps)
+        sysm.close()
+        mode = "serial" if used == 1 else f"OpenMP, {used} threads"
+        out.append({
+            "value": round(cfg.n * steps / t / 1e6, 3),
+            "unit": "MLUPS",
+            "cores": used,
+            "kind": "port",
+            "sample": f"{cfg.meta['workload']} {cfg.nx}x{cfg.ny}x{cfg.nz}, {'FP64' if oprec == 8 else 'FP32'} oracle "
+                      f"({mode}), {steps} steps after 1 untimed step ({t:.1f} s), host '{model}' ({ncpu} logical CPUs)",
+        })
+    R.lib().orc_set_threads(0)
+    return out[0], out[1]
 
 
 def stream_probe(gib=2, reps=10):
@@ -346,7 +355,7 @@ def main():
     if not args.no_large_box and args.workload != "C3" and dims is None:
         line["large_box"] = large_box_timing()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
+        line["cpu_baseline"], line["cpu_baseline_all_cores"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

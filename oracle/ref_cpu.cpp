@@ -6,6 +6,9 @@
 //   NSLattice.hpp:188-209  PreCollision BCs -> (D2D copy) -> collideStream -> PostStreaming BCs -> swap
 //   ADLattice.hpp:144-162  PreCollision BCs -> collideStream -> PostStreaming BCs -> swap
 // The D2D copy f -> f_new (NSLattice.hpp:198) is omitted: every collide writes all cells.
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -260,6 +263,19 @@ template <class T> void from_t(const std::vector<T> &v, double *p) {
 extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
+
+// Threads of the per-cell loops (n <= 0: the OpenMP default the process started with; 1 in a build
+// without OpenMP); results do not depend on it.  Returns the count now in effect.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    static const int initial = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : initial);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 void *orc_create(const orc_config *cfg, int prec) {
     SysBase *s = nullptr;

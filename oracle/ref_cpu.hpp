@@ -1,10 +1,11 @@
 // ref_cpu.hpp -- CPU oracle for FluidMaxxer's coupled D3Q19 + D3Q7 step.
 //
 // TEST INFRASTRUCTURE ONLY. This file restates, kernel by kernel, the
-// reference's CUDA update in plain serial C++ (array-of-structures layout
+// reference's CUDA update in plain C++ (array-of-structures layout
 // f[cell*Q + q], x-fastest cells, the reference's expression order), templated
-// on T in {float, double}. It is the checker for the HIP product path and the
-// serial CPU baseline; it is never linked into, loaded by or called from the
+// on T in {float, double}; serial, or OpenMP-parallel over independent cells with
+// identical results. It is the checker for the HIP product path and the CPU
+// baseline; it is never linked into, loaded by or called from the
 // product library.  Only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg may use it.
 //
@@ -21,6 +22,18 @@
 //     series have no reference counterpart; they follow their published / stated
 //     formulas and are pinned by limits (see the test suite).
 #pragma once
+
+// Built with OpenMP, the per-cell loops of the update run in parallel: each cell of such a loop
+// writes only its own outputs and reads nothing another cell of the same loop writes, so the
+// results do not depend on the thread count (the update has no reductions).  The boundary loops
+// that read a neighbour in the array they write (pressure outlet, Neumann copy) and the outlet
+// statistics stay serial.
+#ifdef _OPENMP
+#define ORC_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PAR_FOR
+#endif
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -83,6 +96,7 @@ struct Grid {
 // k_init_eq_d3q19_1d (BGKCollision.cu:44-71): feq with u = 0.
 template <class T> void ns_init_eq_bgk(T *f, const Grid &g, T rho0) {
     const T ux = T(0), uy = T(0), uz = T(0), u2 = T(0);
+    ORC_PAR_FOR
     for (i64 c = 0; c < g.n(); ++c)
         for (int q = 0; q < 19; ++q) {
             const T eu = T(D3Q19::cx[q]) * ux + T(D3Q19::cy[q]) * uy + T(D3Q19::cz[q]) * uz;
@@ -93,6 +107,7 @@ template <class T> void ns_init_eq_bgk(T *f, const Grid &g, T rho0) {
 
 // k_init_eq_d3q19 (PorousBGKCollision.cu:43-58): f = w * rho0.
 template <class T> void ns_init_eq_porous(T *f, const Grid &g, T rho0) {
+    ORC_PAR_FOR
     for (i64 c = 0; c < g.n(); ++c)
         for (int q = 0; q < 19; ++q) f[c * 19 + q] = D3Q19::w<T>(q) * rho0;
 }
@@ -107,6 +122,7 @@ template <class T> T ns_omega(T tau) {
 // k_bgk_stream_pull_d3q19 (BGKCollision.cu:74-177).
 template <class T>
 void ns_collide_bgk(const T *f, T *f_new, const std::uint8_t *obst, const Grid &g, T omega, T rho0) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         const i64 base = cell * 19;
         if (obst && obst[cell]) {  // solid: on-site reverse (:90-97)
@@ -156,6 +172,7 @@ void ns_collide_bgk(const T *f, T *f_new, const std::uint8_t *obst, const Grid &
 // k_porous_bgk_stream_pull_d3q19 (PorousBGKCollision.cu:64-155).
 template <class T>
 void ns_collide_porous(const T *f, T *f_new, const T *porosity, const Grid &g, T omega, T rho0) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         const i64 base = cell * 19;
         int x, y, z;
@@ -207,6 +224,7 @@ void ns_collide_porous(const T *f, T *f_new, const T *porosity, const Grid &g, T
 // domain, no obstacle test); the density fallback is BGKCollision's.
 template <class T>
 void ns_collide_pbb(const T *f, T *f_new, const T *ns_field, const Grid &g, T omega, T rho0) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         const i64 base = cell * 19;
         int x, y, z;
@@ -251,6 +269,7 @@ void ns_collide_pbb(const T *f, T *f_new, const T *ns_field, const Grid &g, T om
 // ------------------------------------------------------------- NS moments
 // k_compute_macros_d3q19 (Macros.cu:20-61).
 template <class T> void ns_moments(const T *f, T *rho, T *ux, T *uy, T *uz, const Grid &g) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         const i64 base = cell * 19;
         T r = T(0), vx = T(0), vy = T(0), vz = T(0);
@@ -353,6 +372,7 @@ void ns_bc_velocity_neq(T *f, const std::vector<i64> &bnd, const std::vector<i64
 // k_velocity_dirichlet_eq_pre_d3q19 (VelocityDirichlet.cu:54-84), on f, PreCollision.
 template <class T>
 void ns_bc_velocity_eq(T *f, const std::uint8_t *mask, const Grid &g, T rho0, T ux, T uy, T uz) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         if (!mask[cell]) continue;
         const i64 base = cell * 19;
@@ -400,6 +420,7 @@ template <class T> void ns_bc_pressure(T *f, const std::uint8_t *mask, const Gri
 
 // BounceBack (BounceBack.cu:12-31) as a true swap of each opposite pair.
 template <class T> void ns_bc_bounceback(T *f, const std::uint8_t *mask, const Grid &g) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < g.n(); ++cell) {
         if (!mask[cell]) continue;
         const i64 base = cell * 19;
@@ -412,6 +433,7 @@ template <class T> void ns_bc_bounceback(T *f, const std::uint8_t *mask, const G
 // ------------------------------------------------------------------ AD
 // k_ad_init_eq_d3q7 (ADBGKCollision.cu:29-38).
 template <class T> void ad_init_eq(T *g, const Grid &gr, T phi0) {
+    ORC_PAR_FOR
     for (i64 c = 0; c < gr.n(); ++c)
         for (int q = 0; q < 7; ++q) g[c * 7 + q] = phi0 * D3Q7::w<T>(q);
 }
@@ -422,6 +444,7 @@ template <class T> void ad_init_eq(T *g, const Grid &gr, T phi0) {
 template <class T>
 void ad_collide(const T *g, T *g_new, T *phi_out, const T *tau, const T *ux, const T *uy, const T *uz,
                 const T *S, const Grid &gr, T phiMin) {
+    ORC_PAR_FOR
     for (i64 cell = 0; cell < gr.n(); ++cell) {
         int x, y, z;
         gr.coords(cell, x, y, z);
@@ -452,6 +475,7 @@ void ad_collide(const T *g, T *g_new, T *phi_out, const T *tau, const T *ux, con
 // k_ad_dirichlet (ADDirichlet.cu:11-30) and k_ad_dirichlet_tanh_pulse_d3q7
 // (ADDirichletTanhPulse.cu:66-86): g <- phi * w on the mask.
 template <class T> void ad_bc_dirichlet(T *g, const std::uint8_t *mask, const Grid &gr, T phi) {
+    ORC_PAR_FOR
     for (i64 c = 0; c < gr.n(); ++c) {
         if (!mask[c]) continue;
         for (int q = 0; q < 7; ++q) g[c * 7 + q] = phi * D3Q7::w<T>(q);
@@ -495,6 +519,7 @@ template <class T> void ad_bc_neumann(T *g, const std::uint8_t *mask, const Grid
 // k_henry_ldf_source (LinearLDF.cu:13-48).
 template <class T>
 void ldf_linear(i64 n, const std::uint8_t *region, const T *phi, T *S, T *q, T K, T kLDF, T phiMin) {
+    ORC_PAR_FOR
     for (i64 i = 0; i < n; ++i) {
         if (!region[i]) {
             S[i] = T(0);
@@ -514,6 +539,7 @@ void ldf_linear(i64 n, const std::uint8_t *region, const T *phi, T *S, T *q, T K
 // k_langmuir_ldf_source (LangmuirLDF.cu:7-41).
 template <class T>
 void ldf_langmuir(i64 n, const std::uint8_t *region, const T *phi, T *S, T *q, T qMax, T b, T kLDF, T phiMin) {
+    ORC_PAR_FOR
     for (i64 i = 0; i < n; ++i) {
         if (!region[i]) {
             S[i] = T(0);

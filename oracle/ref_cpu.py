@@ -85,6 +85,8 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         vp, dp, u8p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint8)
         L.orc_last_error.restype = C.c_char_p
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_create.restype = vp
         L.orc_create.argtypes = [C.POINTER(_Config), C.c_int]
         L.orc_destroy.argtypes = [vp]

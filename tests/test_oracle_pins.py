@@ -451,3 +451,45 @@ def test_tanh_pulse_and_series_values():
     assert R.series_value(8, 20, knots, 10.0) == 0.5
     assert np.isclose(R.series_value(8, 15, knots, 10.0), 0.75)
     assert R.series_value(8, 1000, knots, 10.0) == 0.25
+
+
+# ------------------------------------------------------ OpenMP build of the oracle
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("variant", ["bgk_neq_dirichlet", "pbb_series", "porous_eq"])
+def test_oracle_threads_bit_identical(prec, variant):
+    """The oracle's parallel per-cell loops give the same bits on one thread and on several (every
+    cell of a parallel loop writes only its own outputs), inlet and outlet planes included."""
+    from fluidmaxxer_amd.coupled import CoupledConfig
+    from fluidmaxxer_amd import workloads
+    from tests.lattice import oracle_setup
+
+    nx, ny, nz = 20, 9, 7
+    n = nx * ny * nz
+    rng = np.random.default_rng(17)
+    x = np.arange(n) % nx
+    solid = ((rng.random(n) < 0.25) & (x > 2) & (x < nx - 3)).astype(np.uint8)
+    cfg = CoupledConfig(nx, ny, nz, solid=solid, region=(rng.random(n) < 0.5).astype(np.uint8), tau=0.7,
+                        u_in=(0.05, 0.0, 0.0), tau_ad=0.65, isotherm="langmuir", iso_a=2.0, iso_b=5.0, k_ldf=0.05,
+                        ns_inlet="neq")
+    if variant == "pbb_series":
+        cfg.ns_op, cfg.ns_inlet = "pbb", "eq"
+        cfg.field = np.clip(solid + rng.uniform(-0.05, 0.05, n), 0, 1)
+        cfg.solid = np.zeros_like(solid)
+        cfg.ad_inlet = "series"
+        cfg.inlet_series = workloads.series_per_step(workloads.series_knots(6, 3), 60)
+    elif variant == "porous_eq":
+        cfg.ns_op, cfg.ns_inlet = "porous", "eq"
+        cfg.field = np.where(solid == 1, 0.6, 0.95)
+    outlet = (x == nx - 1).astype(np.uint8)
+    results = []
+    try:
+        for threads in (1, 4):
+            assert R.lib().orc_set_threads(threads) in (1, threads)
+            sysm = R.System(oracle_setup(cfg, R), prec)
+            c, _, _, _ = sysm.run(40, outlet)
+            results.append([c] + [sysm.get(fid) for fid in (R.F, R.G, R.PHI, R.Q)])
+            sysm.close()
+    finally:
+        R.lib().orc_set_threads(0)
+    for a, b in zip(*results):
+        assert np.array_equal(a, b)
