@@ -106,6 +106,7 @@ _SIGS = {
     "fmx_ad_collide_stream": ([_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _GP, _d, _vp], _i),
     "fmx_ad_bc_dirichlet": ([_i, _vp, _vp, _i64, _d, _i64, _vp], _i),
     "fmx_ad_bc_neumann": ([_i, _vp, _vp, _vp, _i64, _i64, _vp], _i),
+    "fmx_ad_bc_dirichlet_series": ([_i, _vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "fmx_ldf_linear": ([_i, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _vp], _i),
     "fmx_ldf_langmuir": ([_i, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp], _i),
     "fmx_coupled_validate": ([_DP, _BP], _i),
@@ -114,6 +115,7 @@ _SIGS = {
     "fmx_coupled_load": ([_DP, _BP, _i, _vp, _vp, _vp, _vp, _vp], _i),
     "fmx_coupled_prologue": ([_DP, _BP, _i, _vp], _i),
     "fmx_coupled_run": ([_DP, _BP, _i64, C.c_int32, _i, _vp], _i),
+    "fmx_coupled_step": ([_DP, _BP, _i64, _i, _vp], _i),
     "fmx_coupled_view_f": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_coupled_view_g": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_outlet_mean": ([_vp, _i64, _i64, _vp, _vp], _i),
@@ -122,9 +124,13 @@ _SIGS = {
     "fmx_adjoint_forward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _i, _vp, _i, _vp], _i),
     "fmx_adjoint_seg_slots": ([C.c_int32, _i, _i], C.c_int64),
     "fmx_adjoint_workspace_bytes": ([_DP], C.c_int64),
+    "fmx_checkpoint_save": ([_DP, _BP, _i, _vp, _i, _i64, _vp], _i),
+    "fmx_checkpoint_load": ([_DP, _BP, _i, _vp, _i, _i64, _vp], _i),
+    "fmx_adjoint_step": ([_DP, _BP, _i, _i64, _i, _i, C.POINTER(AdjointBufs), _vp], _i),
     "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _i, _vp, _i64, C.POINTER(AdjointBufs), _vp],
                              _i),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
+    "fmx_reduce_outlet_mean": ([_i, _vp, _vp, _i64, _vp, _vp], _i),
     "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,
                               C.c_int64, _vp, _vp, C.POINTER(C.c_double)], _i),

@@ -606,6 +606,18 @@ template <int P> static int save_state(const fmx_coupled_bufs &b, int cur, const
     return check_hip(hipMemcpyAsync(dst.phi, b.phi[cur], n * sizeof(C), hipMemcpyDeviceToDevice, s), "save phi");
 }
 
+template <int P> static int restore_state(const fmx_coupled_bufs &b, int cur, const StateView<P> &src, i64 n,
+                                          hipStream_t s) {
+    using S = typename Storage<P>::S;
+    using C = typename Storage<P>::C;
+    if (int rc = check_hip(hipMemcpyAsync(b.f[cur], src.f, 19 * n * sizeof(S), hipMemcpyDeviceToDevice, s), "load f"))
+        return rc;
+    if (int rc = check_hip(hipMemcpyAsync(b.g[cur], src.g, 7 * n * sizeof(S), hipMemcpyDeviceToDevice, s), "load g"))
+        return rc;
+    if (int rc = check_hip(hipMemcpyAsync(b.q, src.q, n * sizeof(C), hipMemcpyDeviceToDevice, s), "load q")) return rc;
+    return check_hip(hipMemcpyAsync(b.phi[cur], src.phi, n * sizeof(C), hipMemcpyDeviceToDevice, s), "load phi");
+}
+
 // The current FP32 state of b into a 16-bit checkpoint slot.
 template <int CP>
 static int save_half_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, const StateView<CP> &dst,
@@ -860,6 +872,47 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
     return after_launch("k_iso_final");
 }
 
+// One reverse step on the state held in b (f[cur], g[cur], q, phi[cur]: the state at the start of
+// `step`), for callers that keep their own checkpoints: the kernel and sums of the sweep in
+// backward(), with outlet_weight indexed by the step itself; grad_iso is rewritten from the running
+// per-block sums after every step.
+template <int P>
+static int single_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, i64 step, int lam_cur,
+                       bool first, const fmx_adjoint_bufs &adj, hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    const CoupledParams<C> prm = make_params<C>(d);
+    const int nblocks = int(coupled_blocks(prm.gx, prm.gy, prm.gz));
+    double *iso_part = static_cast<double *>(adj.workspace);
+    if (first) {  // the terminal condition, and the accumulators cleared
+        if (int rc = clear_frozen(d, b.links, s)) return rc;
+        struct Z {
+            void *p;
+            size_t bytes;
+        } zeros[] = {{adj.lam_f[lam_cur], size_t(19 * n) * sizeof(C)},
+                     {adj.lam_g[lam_cur], size_t(7 * n) * sizeof(C)},
+                     {adj.lam_q, size_t(n) * sizeof(C)},
+                     {adj.lam_phi, size_t(n) * sizeof(C)},
+                     {adj.grad_field, adj.grad_field ? size_t(n) * sizeof(double) : 0},
+                     {iso_part, 2 * size_t(nblocks) * sizeof(double)}};
+        for (const Z &zz : zeros)
+            if (zz.p && zz.bytes)
+                if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_step: clear")) return rc;
+    }
+    const StateView<P> st{static_cast<S *>(b.f[cur]), static_cast<S *>(b.g[cur]), static_cast<C *>(b.q),
+                          static_cast<C *>(b.phi[cur])};
+    const bool want_series = d.ad_inlet == 2 && adj.grad_series;
+    double *row = want_series ? iso_part + 2 * i64(nblocks) : nullptr;
+    if (int rc = adjoint_step<P>(d, b, st, adj, prm, step, step, lam_cur, iso_part, row, s)) return rc;
+    if (want_series) {
+        hipLaunchKernelGGL(k_series_rows, dim3(1), dim3(256), 0, s, row, nblocks, adj.grad_series, step);
+        if (int rc = after_launch("k_series_rows")) return rc;
+    }
+    hipLaunchKernelGGL(k_iso_final, dim3(1), dim3(1024), 0, s, iso_part, nblocks, adj.grad_iso);
+    return after_launch("k_iso_final");
+}
+
 }  // namespace fmx
 
 using namespace fmx;
@@ -912,6 +965,79 @@ int64_t fmx_adjoint_workspace_bytes(const fmx_coupled_desc *d) {
 int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level) {
     if (k < 1) return -1;
     return two_level ? seg_slots_min(k, recast != 0) : seg_slots_single(k, recast != 0);
+}
+
+int fmx_checkpoint_save(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, void *ckpt, int ckpt_prec,
+                        int64_t slot, void *stream) {
+    FMX_TRY_BEGIN
+    if (int rc = fmx_coupled_validate(d, b)) return rc;
+    if ((cur != 0 && cur != 1) || !ckpt || slot < 0) return set_error(FMX_ERR_ARG, "fmx_checkpoint_save: bad arguments");
+    if (!ckpt_prec_ok(d->prec, ckpt_prec))
+        return set_error(FMX_ERR_UNSUPPORTED, "fmx_checkpoint_save: checkpoint precision %d with storage precision %d",
+                         ckpt_prec, d->prec);
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return save_ckpt<FMX_F32>(*d, *b, cur, ckpt, ckpt_prec, slot, n, s);
+        case FMX_F64: return save_ckpt<FMX_F64>(*d, *b, cur, ckpt, ckpt_prec, slot, n, s);
+        case FMX_F16: return save_ckpt<FMX_F16>(*d, *b, cur, ckpt, ckpt_prec, slot, n, s);
+        default: return save_ckpt<FMX_BF16>(*d, *b, cur, ckpt, ckpt_prec, slot, n, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_checkpoint_load(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, const void *ckpt, int ckpt_prec,
+                        int64_t slot, void *stream) {
+    FMX_TRY_BEGIN
+    if (int rc = fmx_coupled_validate(d, b)) return rc;
+    if ((cur != 0 && cur != 1) || !ckpt || slot < 0) return set_error(FMX_ERR_ARG, "fmx_checkpoint_load: bad arguments");
+    if (!ckpt_prec_ok(d->prec, ckpt_prec))
+        return set_error(FMX_ERR_UNSUPPORTED, "fmx_checkpoint_load: checkpoint precision %d with storage precision %d",
+                         ckpt_prec, d->prec);
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    hipStream_t s = as_stream(stream);
+    void *ck = const_cast<void *>(ckpt);
+    // frozen solid cells keep their other phase in f[1 - cur], which a checkpoint does not hold: the
+    // run goes on with every cell on the full update (the same values)
+    if (int rc = clear_frozen(*d, b->links, s)) return rc;
+    if (d->prec == FMX_F32 && ckpt_prec != FMX_F32) {
+        const StateView<FMX_F32> dst{static_cast<float *>(b->f[cur]), static_cast<float *>(b->g[cur]),
+                                     static_cast<float *>(b->q), static_cast<float *>(b->phi[cur])};
+        if (ckpt_prec == FMX_F16) return load_half_ckpt<FMX_F16>(*d, state_at<FMX_F16>(ck, slot, n), dst, n, s);
+        return load_half_ckpt<FMX_BF16>(*d, state_at<FMX_BF16>(ck, slot, n), dst, n, s);
+    }
+    switch (d->prec) {
+        case FMX_F32: return restore_state<FMX_F32>(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
+        case FMX_F64: return restore_state<FMX_F64>(*b, cur, state_at<FMX_F64>(ck, slot, n), n, s);
+        case FMX_F16: return restore_state<FMX_F16>(*b, cur, state_at<FMX_F16>(ck, slot, n), n, s);
+        default: return restore_state<FMX_BF16>(*b, cur, state_at<FMX_BF16>(ck, slot, n), n, s);
+    }
+    FMX_TRY_END
+}
+
+int fmx_adjoint_step(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, int64_t step, int lam_cur,
+                     int first, const fmx_adjoint_bufs *adj, void *stream) {
+    FMX_TRY_BEGIN
+    if (int rc = adjoint_check(d, b, step, 1, 1, d ? d->prec : 0, "fmx_adjoint_step")) return rc;
+    if ((cur != 0 && cur != 1) || (lam_cur != 0 && lam_cur != 1) || !adj)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_step: bad arguments");
+    if (!adj->lam_f[0] || !adj->lam_f[1] || !adj->lam_g[0] || !adj->lam_g[1] || !adj->lam_q || !adj->lam_phi ||
+        !adj->grad_iso)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_step: missing adjoint buffer");
+    const i64 wb = workspace_bytes(*d);
+    if (!adj->workspace || adj->workspace_bytes < wb)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_step: workspace of %lld bytes given, %lld needed",
+                         (long long)(adj->workspace ? adj->workspace_bytes : 0), (long long)wb);
+    if (d->ad_inlet != 2 && adj->grad_series)
+        return set_error(FMX_ERR_ARG, "fmx_adjoint_step: grad_series needs the per-step inlet series (ad_inlet 2)");
+    hipStream_t s = as_stream(stream);
+    switch (d->prec) {
+        case FMX_F32: return single_step<FMX_F32>(*d, *b, cur, step, lam_cur, first != 0, *adj, s);
+        case FMX_F64: return single_step<FMX_F64>(*d, *b, cur, step, lam_cur, first != 0, *adj, s);
+        case FMX_F16: return single_step<FMX_F16>(*d, *b, cur, step, lam_cur, first != 0, *adj, s);
+        default: return single_step<FMX_BF16>(*d, *b, cur, step, lam_cur, first != 0, *adj, s);
+    }
+    FMX_TRY_END
 }
 
 int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,

@@ -496,6 +496,10 @@ int fmx_coupled_view_g(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int
     FMX_TRY_END
 }
 
+int fmx_coupled_step(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step, int cur, void *stream) {
+    return fmx_coupled_run(d, b, step, 1, cur, stream);
+}
+
 int fmx_outlet_mean(const double *acc, int64_t nsteps, int64_t count, double *mean, void *stream) {
     FMX_TRY_BEGIN
     if (nsteps == 0) return FMX_OK;

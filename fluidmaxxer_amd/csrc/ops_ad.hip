@@ -63,6 +63,53 @@ __global__ void __launch_bounds__(256) k_ad_bc_dirichlet(T *__restrict__ g, cons
     for (int q = 0; q < 7; ++q) g[q * n + c] = phi * w7<T>(q);
 }
 
+// ADDirichlet with the value of the step read on the device: the per-step inlet concentration series
+// (ADDirichletTanhPulse.cu:66-100 computes phi_in(s) in double on the host and casts it to T)
+template <class T>
+__global__ void __launch_bounds__(256) k_ad_bc_dirichlet_series(T *__restrict__ g, const i64 *__restrict__ cells,
+                                                                i64 count, i64 n, const double *__restrict__ series,
+                                                                i64 step) {
+    const i64 k = gtid();
+    if (k >= count) return;
+    const i64 c = cells[k];
+    const T phi = T(series[step]);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) g[q * n + c] = phi * w7<T>(q);
+}
+
+// OutletCSV::log's statistics (OutletCSV.hpp:29-58) on the device: FP64 mean, min and max of phi over
+// the listed cells, in a fixed summation order (one workgroup of 1024).
+template <class T>
+__global__ void __launch_bounds__(1024) k_outlet_stats(const T *__restrict__ phi, const i64 *__restrict__ cells,
+                                                       i64 count, double *__restrict__ out) {
+    __shared__ double rs[1024], rmn[1024], rmx[1024];
+    const int t = int(threadIdx.x);
+    double s = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
+    for (i64 k = t; k < count; k += blockDim.x) {
+        const double v = double(phi[cells[k]]);
+        s += v;
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    rs[t] = s;
+    rmn[t] = mn;
+    rmx[t] = mx;
+    __syncthreads();
+    for (int h = int(blockDim.x) / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            rs[t] += rs[t + h];
+            rmn[t] = rmn[t + h] < rmn[t] ? rmn[t + h] : rmn[t];
+            rmx[t] = rmx[t + h] > rmx[t] ? rmx[t + h] : rmx[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = rs[0] / double(count);
+        out[1] = rmn[0];
+        out[2] = rmx[0];
+    }
+}
+
 // k_ad_neumann (ADNeumannOutlet.cu:5-31)
 template <class T>
 __global__ void __launch_bounds__(256) k_ad_bc_neumann(T *__restrict__ g, const i64 *__restrict__ cells,
@@ -300,6 +347,40 @@ int fmx_ad_bc_dirichlet(int prec, void *g, const int64_t *cells, int64_t count, 
                            cells, count, ncells, phi);
     else return unsupported_prec("fmx_ad_bc_dirichlet", prec);
     return after_launch("k_ad_bc_dirichlet");
+    FMX_TRY_END
+}
+
+int fmx_ad_bc_dirichlet_series(int prec, void *g, const int64_t *cells, int64_t count, const double *series,
+                               int64_t step, int64_t ncells, void *stream) {
+    FMX_TRY_BEGIN
+    if (count == 0) return FMX_OK;
+    if (!g || !cells || !series || count < 0 || ncells <= 0 || step < 0)
+        return set_error(FMX_ERR_ARG, "fmx_ad_bc_dirichlet_series: bad arguments");
+    hipStream_t s = as_stream(stream);
+    if (prec == FMX_F32)
+        hipLaunchKernelGGL(k_ad_bc_dirichlet_series<float>, dim3(blocks_for(count)), dim3(256), 0, s,
+                           static_cast<float *>(g), cells, count, ncells, series, step);
+    else if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_ad_bc_dirichlet_series<double>, dim3(blocks_for(count)), dim3(256), 0, s,
+                           static_cast<double *>(g), cells, count, ncells, series, step);
+    else return unsupported_prec("fmx_ad_bc_dirichlet_series", prec);
+    return after_launch("k_ad_bc_dirichlet_series");
+    FMX_TRY_END
+}
+
+int fmx_reduce_outlet_mean(int prec, const void *phi, const int64_t *cells, int64_t count, double *out, void *stream) {
+    FMX_TRY_BEGIN
+    if (!phi || !cells || count <= 0 || !out) return set_error(FMX_ERR_ARG, "fmx_reduce_outlet_mean: bad arguments");
+    hipStream_t s = as_stream(stream);
+    // the concentration is in the compute precision: double for FP64 storage, float otherwise
+    if (prec == FMX_F64)
+        hipLaunchKernelGGL(k_outlet_stats<double>, dim3(1), dim3(1024), 0, s, static_cast<const double *>(phi), cells,
+                           count, out);
+    else if (prec == FMX_F32 || prec == FMX_F16 || prec == FMX_BF16)
+        hipLaunchKernelGGL(k_outlet_stats<float>, dim3(1), dim3(1024), 0, s, static_cast<const float *>(phi), cells,
+                           count, out);
+    else return set_error(FMX_ERR_ARG, "fmx_reduce_outlet_mean: unknown precision %d", prec);
+    return after_launch("k_outlet_stats");
     FMX_TRY_END
 }
 

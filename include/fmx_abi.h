@@ -128,6 +128,11 @@ int fmx_ad_collide_stream(int prec, const void *g, void *g_new, void *phi_out, c
  * ADDirichletTanhPulse.cu:66-155): g <- phi * w on the listed cells */
 int fmx_ad_bc_dirichlet(int prec, void *g, const int64_t *cells, int64_t count, double phi, int64_t ncells,
                         void *stream);
+/* ADDirichletTanhPulse (pre, on g: ADDirichletTanhPulse.cu:66-155) with the per-step inlet value on
+ * the device: g <- T(series[step]) * w on the listed cells (the pulse, or any inlet concentration
+ * curve, tabulated per step; the host computes nothing per step) */
+int fmx_ad_bc_dirichlet_series(int prec, void *g, const int64_t *cells, int64_t count, const double *series,
+                               int64_t step, int64_t ncells, void *stream);
 /* ADNeumannOutlet::apply (ADNeumannOutlet.cu:5-53): g_new[cells[k]] <- g_new[src[k]], all 7 directions */
 int fmx_ad_bc_neumann(int prec, void *g_new, const int64_t *cells, const int64_t *src, int64_t count, int64_t ncells,
                       void *stream);
@@ -210,6 +215,8 @@ int fmx_coupled_prologue(const fmx_coupled_desc *desc, const fmx_coupled_bufs *b
  * buffer (cur + k + 1) % 2, so the result is in (cur + nsteps) % 2. */
 int fmx_coupled_run(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
                     int cur, void *stream);
+/* One step of fmx_coupled_run (step0 = step, nsteps = 1): the loop body of main.cpp:319-357 */
+int fmx_coupled_step(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step, int cur, void *stream);
 /* The NS populations of f[cur] as the reference holds them (x = 0 plane from inlet_raw), converted to
  * compute precision (float or double) SoA in f_out (19*N). */
 int fmx_coupled_view_f(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *f_out,
@@ -282,12 +289,35 @@ int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level);
 int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int64_t step0, int32_t nsteps,
                          int32_t k, const void *ckpt, int ckpt_prec, void *seg, int64_t seg_slots,
                          const fmx_adjoint_bufs *adj, void *stream);
+/* Checkpoints of a run (no counterpart in the reference, which keeps no run state): slot `slot` of
+ * ckpt (slots of fmx_adjoint_ckpt_bytes(desc, ckpt_prec)) <- the fused state f[cur], g[cur], q,
+ * phi[cur], as stored or (FP32 run) as FP16 / BF16 deviations.  fmx_checkpoint_load puts it back
+ * (a 16-bit slot decoded to FP32) and clears the frozen-solid link bits, so fmx_coupled_run goes on
+ * from it as from the step it was saved before (exactly, for a checkpoint at the storage precision). */
+int fmx_checkpoint_save(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *ckpt, int ckpt_prec,
+                        int64_t slot, void *stream);
+int fmx_checkpoint_load(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, const void *ckpt,
+                        int ckpt_prec, int64_t slot, void *stream);
+/* One reverse step (no counterpart in the reference), for callers that keep their own checkpoints:
+ * bufs holds the state at the start of `step` (f[cur], g[cur], q, phi[cur]; fmx_checkpoint_load)
+ * and adj->lam_f / lam_g[lam_cur] the adjoint of step + 1's pulls; writes the adjoint of this step's
+ * pulls into lam_f / lam_g[1 - lam_cur], updates lam_q / lam_phi, adds into grad_field and (ad_inlet
+ * 2) grad_series[step], and rewrites grad_iso with the running total.  The loss weight of this step
+ * is adj->outlet_weight[step] (indexed by the step itself).  first = 1 for the last step of the run:
+ * sets the terminal condition (lam_*[lam_cur], lam_q, lam_phi = 0) and clears grad_field and the
+ * running sums; grad_series is not cleared.  Steps taken last to first, lam_cur alternating, give
+ * fmx_adjoint_backward's gradients to the bit. */
+int fmx_adjoint_step(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, int64_t step, int lam_cur,
+                     int first, const fmx_adjoint_bufs *adj, void *stream);
 
 /* ---------------------------------------------------------- diagnostics */
 /* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);
  * f is SoA in `prec` storage (half storage needs f_ref); any input may be NULL (sum = 0) */
 int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
                     void *stream);
+/* OutletCSV::log (OutletCSV.hpp:29-58) on the device: out[0] = mean, out[1] = min, out[2] = max of phi
+ * (compute precision of `prec`) over the listed cells, in FP64 and a fixed summation order */
+int fmx_reduce_outlet_mean(int prec, const void *phi, const int64_t *cells, int64_t count, double *out, void *stream);
 
 /* Streaming-bandwidth probes: mode 0 copies `bytes` from src to dst (16 B per lane, non-temporal),
  * mode 1 only reads src (dst: a 4-byte sink); modes 2 / 3 / 4 copy 26 planes of n = bytes / (26 * e)

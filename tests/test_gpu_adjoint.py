@@ -344,6 +344,43 @@ def test_adjoint_launch_variants(pbb_problem, monkeypatch, env):
         assert np.max(np.abs(g2["series"] - g["series"])) <= 1e-12 * np.max(np.abs(g["series"]))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_adjoint_step_matches_sweep(pbb_problem, prec):
+    """Reverse steps taken one at a time (fmx_checkpoint_load of every stored state, then
+    fmx_adjoint_step, last step first) give fmx_adjoint_backward's gradients to the bit."""
+    import ctypes
+    import torch
+
+    from fluidmaxxer_amd import abi
+    pr = Problem(adj_config("pbb"), prec=prec, k=1, target=pbb_problem.target)
+    g = pr.grads()
+    run, sim = pr.run, pr.run.sim
+    w = 2.0 * (host(run.forward()) - pr.target)  # every state before steps 1 .. NSTEPS in ckpt slots 0 ..
+    ow = torch.zeros(NSTEPS + 1, dtype=torch.float64, device="cuda")
+    ow[1:] = torch.as_tensor(w, device="cuda")  # indexed by the step itself
+    gi = torch.zeros(2, dtype=torch.float64, device="cuda")
+    gf = torch.zeros_like(run.grad_field)
+    gs = torch.zeros_like(run.grad_series)
+    a = abi.AdjointBufs()
+    a.lam_f[0], a.lam_f[1] = run.lam_f[0].data_ptr(), run.lam_f[1].data_ptr()
+    a.lam_g[0], a.lam_g[1] = run.lam_g[0].data_ptr(), run.lam_g[1].data_ptr()
+    a.lam_q, a.lam_phi = run.lam_q.data_ptr(), run.lam_phi.data_ptr()
+    a.workspace, a.workspace_bytes = run.workspace.data_ptr(), run.workspace.numel()
+    a.outlet_weight, a.grad_iso, a.grad_field, a.grad_series = ow.data_ptr(), gi.data_ptr(), gf.data_ptr(), gs.data_ptr()
+    say(f"  adjoint one step at a time, prec={prec}")
+    lam = 0
+    for j in range(NSTEPS - 1, -1, -1):
+        abi.call("fmx_checkpoint_load", sim._desc_p(), sim._bufs_p(), 0, run.ckpt.data_ptr(), run.ckpt_prec, j,
+                 sim._s())
+        abi.call("fmx_adjoint_step", sim._desc_p(), sim._bufs_p(), 0, 1 + j, lam, int(j == NSTEPS - 1),
+                 ctypes.byref(a), sim._s())
+        lam ^= 1
+    gi, gf, gs = host(gi), host(gf), host(gs)
+    assert gi[0] == g["iso_a"] and gi[1] == g["iso_b"]
+    assert np.array_equal(gf, g["field"])
+    assert np.array_equal(gs, g["series"])
+
+
 def test_c4_recovers_isotherm_constants():
     """The C4 inverse problem as specified: the target curve is the forward run with q_max x 1.1 and
     K_L x 0.9, so the minimum of L = sum_s (C_out(s) - C*(s))^2 is known (0, at those constants).

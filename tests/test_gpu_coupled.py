@@ -327,3 +327,45 @@ def test_frozen_solids_bit_exact(monkeypatch):
     raw = host(sim.links.view(torch_int32())).astype(np.int64) & 0xFFFFFFFF
     x = np.arange(cfg.n) % cfg.nx
     assert np.array_equal(((raw >> 21) & 1) == 1, (cfg.solid == 1) & (x >= 2) & (x <= cfg.nx - 3))
+
+
+@pytest.mark.parametrize("prec,ckpt", [("f32", "f32"), ("f64", "f64"), ("f16", "f16"), ("f32", "f16")])
+def test_coupled_step_and_checkpoints(prec, ckpt):
+    """fmx_coupled_step is one step of fmx_coupled_run; fmx_checkpoint_save / load put a run back to a
+    saved step and the run goes on from it to the same bits (from a 16-bit checkpoint of an FP32 run:
+    within the checkpoint's rounding)."""
+    import torch
+
+    from fluidmaxxer_amd import abi
+    cfg = small_config("langmuir_series")
+    say(f"coupled step / checkpoints prec={prec} ckpt={ckpt}")
+    a, b = CoupledSim(cfg, prec=prec), CoupledSim(cfg, prec=prec)
+    a.launch_only(30)
+    for _ in range(30):
+        abi.call("fmx_coupled_step", b._desc_p(), b._bufs_p(), b.step_index + 1, b.cur, b._s())
+        b.cur ^= 1
+        b.step_index += 1
+
+    def state(s):
+        return [host(t) for t in (s.view_f(), s.view_g(), s.phi, s.q)]
+
+    for x, y in zip(state(a), state(b)):
+        assert np.array_equal(x, y)
+    cp = abi.PREC_NAMES[ckpt]
+    slot_bytes = int(abi.lib().fmx_adjoint_ckpt_bytes(a._desc_p(), cp))
+    store = torch.empty(2 * slot_bytes, dtype=torch.uint8, device="cuda")
+    cur0, step0 = a.cur, a.step_index
+    abi.call("fmx_checkpoint_save", a._desc_p(), a._bufs_p(), a.cur, store.data_ptr(), cp, 1, a._s())
+    a.launch_only(25)
+    ref = state(a)
+    abi.call("fmx_checkpoint_load", a._desc_p(), a._bufs_p(), cur0, store.data_ptr(), cp, 1, a._s())
+    a.cur, a.step_index = cur0, step0
+    a.launch_only(25)
+    got = state(a)
+    if ckpt == prec:
+        for x, y in zip(ref, got):
+            assert np.array_equal(x, y)
+    else:
+        errs = [rel_err(y, x) for x, y in zip(ref, got)]
+        say("  restart from a 16-bit checkpoint:", [f"{e:.1e}" for e in errs])
+        assert max(errs) < 1e-3

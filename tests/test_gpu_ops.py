@@ -245,3 +245,38 @@ def test_init_pack_unpack(prec):
     assert np.allclose(host(aos), np.tile(w * (np.float32(1.3) if prec == abi.F32 else 1.3), n), rtol=1e-7)
     assert np.array_equal(host(back), host(f))
     assert np.allclose(aos_to_soa(soa_to_aos(host(gg), 7), 7).sum(0), 0.25, rtol=1e-7)
+
+
+@pytest.mark.parametrize("prec,oprec,tol", PRECS)
+def test_ad_dirichlet_series_and_outlet_stats(prec, oprec, tol):
+    """The inlet value read on the device from series[step] (ADDirichletTanhPulse with the pulse
+    tabulated per step), and OutletCSV's mean / min / max reduced on the device in FP64."""
+    dims = (8, 5, 4)
+    n = 160
+    say(f"ad_bc series / outlet stats prec={prec}")
+    rng = np.random.default_rng(23)
+    g = rng.random(n * 7)
+    tdt = _tdt(prec)
+    inlet = x_plane(*dims, 0)
+    series = rng.uniform(0.0, 1.0, 12)
+    ci = dev(np.nonzero(inlet)[0].astype(np.int64), torch().int64)
+    sd = dev(series, torch().float64)
+    gd = dev(aos_to_soa(g, 7).ravel(), tdt)
+    abi.call("fmx_ad_bc_dirichlet_series", prec, gd.data_ptr(), ci.data_ptr(), ci.numel(), sd.data_ptr(), 7, n,
+             stream())
+    gc = dev(aos_to_soa(g, 7).ravel(), tdt)
+    abi.call("fmx_ad_bc_dirichlet", prec, gc.data_ptr(), ci.data_ptr(), ci.numel(), float(series[7]), n, stream())
+    sync()
+    assert np.array_equal(host(gd), host(gc))
+    assert rel_err(soa_to_aos(host(gd), 7), R.op_ad_bc(oprec, R.AD_DIRICHLET, dims, g, inlet, a=series[7])) < tol
+    for count in (len(np.nonzero(x_plane(*dims, 7))[0]), 3000):
+        phi = rng.uniform(0.0, 1.0, max(n, count))
+        cells = np.nonzero(x_plane(*dims, 7))[0] if count < n else rng.permutation(phi.size)[:count]
+        pd, cd = dev(phi, tdt), dev(cells.astype(np.int64), torch().int64)
+        out = torch().zeros(3, dtype=torch().float64, device="cuda")
+        abi.call("fmx_reduce_outlet_mean", prec, pd.data_ptr(), cd.data_ptr(), len(cells), out.data_ptr(), stream())
+        sync()
+        vals = host(pd).astype(np.float64)[cells]
+        o = host(out)
+        assert abs(o[0] - vals.mean()) <= 1e-14 * abs(vals.mean()), (o[0], vals.mean())
+        assert o[1] == vals.min() and o[2] == vals.max()
