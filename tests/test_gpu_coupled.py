@@ -190,15 +190,19 @@ def test_fused_step_boundary_writer_fallback(variant):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("dims", [(3, 5, 4), (4, 3, 3), (5, 2, 7)])
-def test_fused_step_small_extents(dims):
-    """Smallest supported boxes: the x = 1 and x = nx-2 threads carry both boundary updates."""
+@pytest.mark.parametrize("dims", [(3, 5, 4), (4, 3, 3), (5, 2, 7), (6, 1, 1), (9, 1, 4), (7, 4, 1)])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_fused_step_small_extents(dims, periodic):
+    """Smallest supported boxes: the x = 1 and x = nx-2 threads carry both boundary updates; with
+    ny or nz = 1 a periodic row is its own y / z neighbour, a walled one keeps its populations.  (The
+    NEQ inlet needs a plane it can tell apart from a y or z plane: VelocityDirichletNEQ.hpp:26-107
+    rejects an x = 0 mask that is also y = const or z = const, so those boxes take the Eq inlet.)"""
     nx, ny, nz = dims
     n = nx * ny * nz
     rng = np.random.default_rng(3)
     cfg = CoupledConfig(nx, ny, nz, solid=(rng.random(n) < 0.2).astype(np.uint8),
                         region=(rng.random(n) < 0.5).astype(np.uint8), tau=0.9, tau_ad=0.8, k_ldf=0.1,
-                        ns_inlet="neq")
+                        ns_inlet="neq" if min(ny, nz) > 1 else "eq", periodic_y=periodic, periodic_z=periodic)
     errs = run_pair(cfg, "f64", 40)
     assert max(errs.values()) < 1e-11, errs
 
