@@ -20,7 +20,7 @@ F32, F64, F16, BF16 = 0, 1, 2, 3
 NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
-OUTLET_SLOTS, OUTLET_STRIDE = 64, 16
+OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
 
@@ -56,6 +56,7 @@ class CoupledBufs(C.Structure):
         ("q", C.c_void_p), ("phi", C.c_void_p * 2),
         ("links", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
         ("inlet_raw", C.c_void_p), ("inlet_series", C.c_void_p), ("outlet_acc", C.c_void_p),
+        ("outlet_ring", C.c_void_p),
     ]
 
 

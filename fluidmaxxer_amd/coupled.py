@@ -129,6 +129,7 @@ class CoupledSim:
         self.cur = 0
         self.step_index = 0
         self._acc = None
+        self._ring = None
         self.stream = torch.cuda.current_stream(dev)
         abi.call("fmx_coupled_build_links", self._desc_p(), self.flags.data_ptr(), self.links.data_ptr(), self._s())
         abi.call("fmx_coupled_init", self._desc_p(), self._bufs_p(), self.cur, float(cfg.phi0), self._s())
@@ -156,6 +157,11 @@ class CoupledSim:
         b.inlet_raw = self.inlet_raw.data_ptr() if self.inlet_raw is not None else None
         b.inlet_series = self.inlet_series.data_ptr() if self.inlet_series is not None else None
         b.outlet_acc = acc.data_ptr() if acc is not None else None
+        if acc is not None:  # outlet sums in a fixed order (repeatable to the bit)
+            if self._ring is None:
+                self._ring = _torch().empty(abi.OUTLET_RING_ROWS * self.cfg.ny * self.cfg.nz,
+                                            dtype=_torch().float64, device=self.device)
+            b.outlet_ring = self._ring.data_ptr()
         return b
 
     def _bufs_p(self, acc=None):

@@ -248,7 +248,26 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     p.inlet_raw = static_cast<C *>(b.inlet_raw);
     p.inlet_series = b.inlet_series;
     p.outlet_acc = b.outlet_acc;
+    p.outlet_ring_row = nullptr;  // set per step by run_coupled_steps
     return p;
+}
+
+// Row r of the outlet ring holds the outlet-plane concentrations of the step whose accumulator row is
+// acc_row0 + r; one workgroup per row adds them in a fixed order (plane index order per thread, then
+// a fixed tree) into slot 0 of that accumulator row.
+__global__ void __launch_bounds__(256) k_outlet_ring_rows(const double *__restrict__ ring, i64 plane,
+                                                          double *__restrict__ acc, i64 acc_row0) {
+    __shared__ double red[256];
+    const double *row = ring + i64(blockIdx.x) * plane;
+    double s = 0.0;
+    for (i64 i = threadIdx.x; i < plane; i += blockDim.x) s += row[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[(acc_row0 + i64(blockIdx.x)) * (FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE)] += red[0];
 }
 
 template int launch_coupled_step<FMX_F32>(int, const CoupledPtrs<FMX_F32> &, const CoupledParams<float> &, i64, i64,
@@ -269,10 +288,20 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
                       hipStream_t s, i64 acc_row0) {
     using C = typename Storage<P>::C;
     const CoupledParams<C> prm = make_params<C>(d);
+    const bool ring = b.outlet_acc && b.outlet_ring;
+    const i64 plane = i64(d.ny) * d.nz;
+    int rows = 0;  // ring rows filled since the last fold
     for (int k = 0; k < nsteps; ++k) {
         CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
         if (k != nsteps - 1) ptr.inlet_raw = nullptr;  // the raw x = 0 plane is state read-back only
+        if (ring) ptr.outlet_ring_row = b.outlet_ring + i64(rows) * plane;
         if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, acc_row0 + k, s)) return rc;
+        if (ring && (++rows == FMX_OUTLET_RING_ROWS || k == nsteps - 1)) {
+            hipLaunchKernelGGL(k_outlet_ring_rows, dim3(rows), dim3(256), 0, s, b.outlet_ring, plane, b.outlet_acc,
+                               acc_row0 + k + 1 - rows);
+            if (int rc = after_launch("k_outlet_ring_rows")) return rc;
+            rows = 0;
+        }
     }
     return FMX_OK;
 }

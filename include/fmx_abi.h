@@ -193,7 +193,12 @@ typedef struct fmx_coupled_bufs {
     double *outlet_acc;       /* nullable: FMX_OUTLET_SLOTS*FMX_OUTLET_STRIDE doubles per step, accumulating
                                  phi over x = nx-1; step s (of a run starting at step0) at
                                  outlet_acc[(s - step0) * SLOTS * STRIDE]; caller zeroes it */
+    double *outlet_ring;      /* nullable scratch of FMX_OUTLET_RING_ROWS*ny*nz doubles: with it, the outlet
+                                 cells store phi there and every step's sum is added in a fixed order (every
+                                 FMX_OUTLET_RING_ROWS steps and at the end of a run), so the breakthrough
+                                 curve repeats to the bit; without it the sums are FP64 atomic adds */
 } fmx_coupled_bufs;
+#define FMX_OUTLET_RING_ROWS 64
 
 int fmx_coupled_validate(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs);
 /* Builds the per-cell link words from the flag bytes (FMX_FLAG_SOLID | FMX_FLAG_REGION per cell)

@@ -373,3 +373,29 @@ def test_coupled_step_and_checkpoints(prec, ckpt):
         errs = [rel_err(y, x) for x, y in zip(ref, got)]
         say("  restart from a 16-bit checkpoint:", [f"{e:.1e}" for e in errs])
         assert max(errs) < 1e-3
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_outlet_sums_repeat_bitwise(prec):
+    """With the outlet ring the breakthrough sums are added in a fixed order: two runs give the same
+    curve to the bit (150 steps: two full folds of the 64-row ring and a partial one), and it equals
+    the FP64 atomic sums of a run without the ring to rounding."""
+    import ctypes
+
+    import torch
+
+    from fluidmaxxer_amd import abi
+    cfg = small_config("eq_inlet", dims=(40, 48, 40))
+    cfg.phi0 = 0.3  # a breakthrough curve well above the clamp floor from the first step
+    say(f"outlet sums prec={prec}")
+    curves = [host(CoupledSim(cfg, prec=prec).run(150)) for _ in range(2)]
+    assert np.array_equal(curves[0], curves[1])
+    sim = CoupledSim(cfg, prec=prec)
+    acc = sim.outlet_acc(150)
+    b = sim._bufs(acc)
+    b.outlet_ring = None
+    abi.call("fmx_coupled_run", sim._desc_p(), ctypes.byref(b), 1, 150, sim.cur, sim._s())
+    mean = torch.empty(150, dtype=torch.float64, device="cuda")
+    abi.call("fmx_outlet_mean", acc.data_ptr(), 150, cfg.ny * cfg.nz, mean.data_ptr(), sim._s())
+    atomic = host(mean)
+    assert np.allclose(curves[0], atomic, rtol=1e-13, atol=0)
