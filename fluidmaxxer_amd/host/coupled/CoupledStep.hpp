@@ -69,6 +69,7 @@ public:
         fmx::freeBytes(d_series_);
         fmx::freeBytes(d_acc_);
         fmx::freeBytes(d_mean_);
+        fmx::freeBytes(d_ring_);
     }
     CoupledStep(const CoupledStep &) = delete;
     CoupledStep &operator=(const CoupledStep &) = delete;
@@ -254,6 +255,11 @@ private:
         bufs_.inlet_raw = d_inlet_raw_;
         bufs_.inlet_series = d_series_;
         bufs_.outlet_acc = acc;
+        if (acc) {  // breakthrough sums in a fixed order (the same curve on every run)
+            if (!d_ring_)
+                d_ring_ = static_cast<double *>(fmx::mallocBytes(sizeof(double) * FMX_OUTLET_RING_ROWS * nyz_));
+            bufs_.outlet_ring = d_ring_;
+        }
         return &bufs_;
     }
 
@@ -336,7 +342,7 @@ private:
     std::function<double(int)> series_;
     double *d_series_ = nullptr;
     int series_cap_ = 0;
-    double *d_acc_ = nullptr, *d_mean_ = nullptr;
+    double *d_acc_ = nullptr, *d_mean_ = nullptr, *d_ring_ = nullptr;
     int acc_cap_ = 0;
     std::vector<double> outlet_;
 };
