@@ -248,7 +248,6 @@ template <int P> static CoupledPtrs<P> make_ptrs(const fmx_coupled_bufs &b, int 
     p.inlet_raw = static_cast<C *>(b.inlet_raw);
     p.inlet_series = b.inlet_series;
     p.outlet_acc = b.outlet_acc;
-    p.outlet_ring_row = nullptr;  // set per step by run_coupled_steps
     return p;
 }
 
@@ -294,8 +293,8 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
     for (int k = 0; k < nsteps; ++k) {
         CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
         if (k != nsteps - 1) ptr.inlet_raw = nullptr;  // the raw x = 0 plane is state read-back only
-        if (ring) ptr.outlet_ring_row = b.outlet_ring + i64(rows) * plane;
-        if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, acc_row0 + k, s)) return rc;
+        if (ring) ptr.outlet_acc = b.outlet_ring + i64(rows) * plane;  // acc_row -1: store into this ring row
+        if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, s)) return rc;
         if (ring && (++rows == FMX_OUTLET_RING_ROWS || k == nsteps - 1)) {
             hipLaunchKernelGGL(k_outlet_ring_rows, dim3(rows), dim3(256), 0, s, b.outlet_ring, plane, b.outlet_acc,
                                acc_row0 + k + 1 - rows);

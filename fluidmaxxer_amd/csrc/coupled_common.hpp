@@ -39,8 +39,7 @@ template <int P> struct CoupledPtrs {
     const C *tau_field;
     C *inlet_raw;   // nullable: x = 0 populations as the reference holds them between steps
     const double *inlet_series;
-    double *outlet_acc;
-    double *outlet_ring_row;  // nullable: this step's row of the outlet ring (plain stores instead of atomics)
+    double *outlet_acc;       // accumulator rows (acc_row >= 0), or this step's outlet-ring row (acc_row < 0)
 };
 
 // Logical block id -> (bx, by, bz).  With xcd_remap the hardware's round-robin

@@ -246,8 +246,8 @@ __global__ void __launch_bounds__(256, P == FMX_F16 ? 8 : (P == FMX_F64 ? 4 : 1)
 
     // ------------------------------------------------- outlet breakthrough sum
     if (p.outlet_acc && x == nx - 1) {
-        if (p.outlet_ring_row) {  // summed later in a fixed order
-            p.outlet_ring_row[unsigned(y) + unsigned(ny) * unsigned(z)] = double(phi);
+        if (acc_row < 0) {  // a row of the outlet ring, summed later in a fixed order
+            p.outlet_acc[unsigned(y) + unsigned(ny) * unsigned(z)] = double(phi);
         } else {
             const int slot = int((unsigned(y) + unsigned(ny) * unsigned(z)) % FMX_OUTLET_SLOTS);
             unsafeAtomicAdd(p.outlet_acc + acc_row * (FMX_OUTLET_SLOTS * FMX_OUTLET_STRIDE) + slot * FMX_OUTLET_STRIDE,
