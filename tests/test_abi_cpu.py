@@ -183,3 +183,32 @@ def test_vti_reader_errors(tmp_path):
         geometry.read_vti(f)
     with pytest.raises(abi.FmxError, match="could not open"):
         geometry.read_vti(str(tmp_path / "missing.vti"))
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/fmx_abi.h is C99 (any FFI can read it) and a C program calling the fused path, the
+    checkpoint and adjoint entry points links against libfmx_hip.so (compile and link only)."""
+    import subprocess
+
+    src = tmp_path / "use_abi.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "fmx_abi.h"
+int main(void) {
+    fmx_coupled_desc d = {0};
+    fmx_coupled_bufs b = {0};
+    fmx_adjoint_bufs a = {0};
+    d.nx = 8; d.ny = 4; d.nz = 4; d.prec = FMX_F32; d.ns_op = FMX_NS_BGK;
+    if (fmx_coupled_validate(&d, &b) == 0) return 1;
+    printf("%s %lld %lld\n", fmx_last_error(), (long long)fmx_adjoint_workspace_bytes(&d),
+           (long long)FMX_OUTLET_RING_ROWS);
+    (void)fmx_coupled_step; (void)fmx_checkpoint_save; (void)fmx_checkpoint_load; (void)fmx_adjoint_step;
+    (void)fmx_reduce_outlet_mean; (void)fmx_ad_bc_dirichlet_series; (void)a;
+    return 0;
+}
+''')
+    exe = tmp_path / "use_abi"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(abi.ROOT, "include"), str(src),
+                        "-o", str(exe), "-L", os.path.dirname(abi.LIB_PATH), "-lfmx_hip"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
