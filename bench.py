@@ -175,6 +175,67 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None, ckpt=None, two_
     return res
 
 
+def adjoint_c5_accuracy(steps=None, k=500):
+    """C5 at its full 256x128x128: the FP32 run with FP16-stored checkpoints every k steps and FP32
+    recompute against the FP64 adjoint (both with two-level segment stores, so the FP64 store fits in
+    about 80 GB); relative l2 errors of dL/d(200 knots), dL/d(solid fraction), the isotherm gradients
+    and C_out.  As in the reduced-size test, the fluid cells' solid fraction and the concentration clamp
+    are 0 (the specified +-0.05 noise stops the flow), and the loss is sum (C_out - 0.85 C_out^FP64)^2."""
+    import numpy as np
+    import torch
+
+    from fluidmaxxer_amd import abi, workloads
+    from fluidmaxxer_amd.adjoint import AdjointRun
+
+    nsteps = steps or workloads.SPECS["C5"]["steps"]
+    cfg = workloads.build("C5", steps=nsteps)
+    cfg.field = np.where(cfg.field > 0.5, cfg.field, 0.0)
+    cfg.min_conc = 0.0
+    knots = np.asarray(cfg.meta["knots"])
+    # d(per-step inlet value)/d(knot) of the per-step linear interpolation
+    s = np.arange(nsteps + 1) * (len(knots) - 1) / nsteps
+    i0 = np.minimum(np.floor(s).astype(int), len(knots) - 2)
+    B = np.zeros((nsteps + 1, len(knots)))
+    B[np.arange(nsteps + 1), i0] = 1 - (s - i0)
+    B[np.arange(nsteps + 1), i0 + 1] = s - i0
+
+    def gradients(prec, ckpt, target_of):
+        seg = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+        run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt, seg_slots=seg)
+        c = run.forward().cpu().numpy()
+        target = target_of(c)
+        t0 = time.perf_counter()
+        g = run.backward(2.0 * (c - target))
+        torch.cuda.synchronize()
+        out = {"c": c, "target": target, "seconds": time.perf_counter() - t0, "iso_a": g["iso_a"],
+               "iso_b": g["iso_b"], "field": g["field"].cpu().numpy(), "knots": B.T @ g["series"].cpu().numpy()}
+        del run
+        torch.cuda.empty_cache()
+        return out
+
+    def rel_l2(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    def errors(a, b):
+        return {"knots": rel_l2(a["knots"], b["knots"]), "field": rel_l2(a["field"], b["field"]),
+                "c_out": rel_l2(a["c"], b["c"]), "iso_a": abs(a["iso_a"] - b["iso_a"]) / abs(b["iso_a"]),
+                "iso_b": abs(a["iso_b"] - b["iso_b"]) / abs(b["iso_b"])}
+
+    g64 = gradients("f64", None, lambda c: 0.85 * c)
+    g16 = gradients("f32", "f16", lambda c: g64["target"])
+    g32 = gradients("f32", None, lambda c: g64["target"])
+    return {
+        "workload": f"C5: {cfg.nx}x{cfg.ny}x{cfg.nz} pbb langmuir, {nsteps} steps; FP32 run, FP16 checkpoints every {k}, "
+                    f"FP32 recompute, against the FP64 adjoint (two-level stores)",
+        "c_out_last": float(g64["c"][-1]),
+        "rel_l2": errors(g16, g64),
+        "rel_l2_fp32_ckpt_vs_fp64": errors(g32, g64),
+        "rel_l2_fp16_ckpt_vs_fp32_ckpt": errors(g16, g32),
+        "backward_seconds": {"f64": round(g64["seconds"], 2), "f32_f16ckpt": round(g16["seconds"], 2),
+                             "f32": round(g32["seconds"], 2)},
+    }
+
+
 def large_box_timing(workload="C3", precs=("f32", "f16", "bf16"), steps=100, warmup=20):
     """The 512^3 Langmuir sphere pack (C3) for each storage precision: step time over `steps` launches
     between two events on the launch stream, MLUPS and the fraction of the algorithmic roofline."""
@@ -236,6 +297,8 @@ def main():
     ap.add_argument("--no-large-box", action="store_true", help="skip the 512^3 (C3) timing per storage precision")
     ap.add_argument("--adjoint-steps", type=int, default=None, metavar="N",
                     help="steps of the adjoint timing (default: the workload's own)")
+    ap.add_argument("--adjoint-c5-accuracy", action="store_true",
+                    help="only C5 at full size: FP16-checkpoint gradients against the FP64 adjoint")
     ap.add_argument("--adjoint-c5", action="store_true",
                     help="only the adjoint/forward timing on C5 as specified (256x128x128, 20000 steps, FP16 "
                          "checkpoints every 500 steps, FP32 recompute; one-level and two-level segment stores)")
@@ -254,6 +317,10 @@ def main():
 
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
+        return
+
+    if args.adjoint_c5_accuracy:
+        print(json.dumps({"adjoint_c5_accuracy": adjoint_c5_accuracy(args.adjoint_steps)}), flush=True)
         return
 
     if args.adjoint_c5:
