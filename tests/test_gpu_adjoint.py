@@ -344,7 +344,7 @@ def test_adjoint_launch_variants(pbb_problem, monkeypatch, env):
         assert np.max(np.abs(g2["series"] - g["series"])) <= 1e-12 * np.max(np.abs(g["series"]))
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 def test_adjoint_step_matches_sweep(pbb_problem, prec):
     """Reverse steps taken one at a time (fmx_checkpoint_load of every stored state, then
     fmx_adjoint_step, last step first) give fmx_adjoint_backward's gradients to the bit."""

@@ -333,7 +333,8 @@ def test_frozen_solids_bit_exact(monkeypatch):
     assert np.array_equal(((raw >> 21) & 1) == 1, (cfg.solid == 1) & (x >= 2) & (x <= cfg.nx - 3))
 
 
-@pytest.mark.parametrize("prec,ckpt", [("f32", "f32"), ("f64", "f64"), ("f16", "f16"), ("f32", "f16")])
+@pytest.mark.parametrize("prec,ckpt", [("f32", "f32"), ("f64", "f64"), ("f16", "f16"), ("bf16", "bf16"), ("f32", "f16"),
+                                       ("f32", "bf16")])
 def test_coupled_step_and_checkpoints(prec, ckpt):
     """fmx_coupled_step is one step of fmx_coupled_run; fmx_checkpoint_save / load put a run back to a
     saved step and the run goes on from it to the same bits (from a 16-bit checkpoint of an FP32 run:
@@ -372,11 +373,11 @@ def test_coupled_step_and_checkpoints(prec, ckpt):
     else:
         errs = [rel_err(y, x) for x, y in zip(ref, got)]
         say("  restart from a 16-bit checkpoint:", [f"{e:.1e}" for e in errs])
-        assert max(errs) < 1e-3
+        assert max(errs) < (1e-3 if ckpt == "f16" else 1e-2)
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_outlet_sums_repeat_bitwise(prec):
+@pytest.mark.parametrize("prec,pair", [("f32", False), ("f64", False), ("f32", True), ("f16", True)])
+def test_outlet_sums_repeat_bitwise(prec, pair, monkeypatch):
     """With the outlet ring the breakthrough sums are added in a fixed order: two runs give the same
     curve to the bit (150 steps: two full folds of the 64-row ring and a partial one), and it equals
     the FP64 atomic sums of a run without the ring to rounding."""
@@ -385,9 +386,11 @@ def test_outlet_sums_repeat_bitwise(prec):
     import torch
 
     from fluidmaxxer_amd import abi
-    cfg = small_config("eq_inlet", dims=(40, 48, 40))
+    if pair:  # the two-cells-per-thread kernel (nx % 128 == 0) writes the ring the same way
+        monkeypatch.setenv("FMX_PAIR", "1")
+    cfg = small_config("eq_inlet", dims=(128, 24, 20) if pair else (40, 48, 40))
     cfg.phi0 = 0.3  # a breakthrough curve well above the clamp floor from the first step
-    say(f"outlet sums prec={prec}")
+    say(f"outlet sums prec={prec} pair={pair}")
     curves = [host(CoupledSim(cfg, prec=prec).run(150)) for _ in range(2)]
     assert np.array_equal(curves[0], curves[1])
     sim = CoupledSim(cfg, prec=prec)
