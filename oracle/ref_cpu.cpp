@@ -6,6 +6,15 @@
 //   NSLattice.hpp:188-209  PreCollision BCs -> (D2D copy) -> collideStream -> PostStreaming BCs -> swap
 //   ADLattice.hpp:144-162  PreCollision BCs -> collideStream -> PostStreaming BCs -> swap
 // The D2D copy f -> f_new (NSLattice.hpp:198) is omitted: every collide writes all cells.
+//
+// orc_set_step_mutation(m != 0) plants one ordering mistake in the step sequence, so the
+// sequence pins of the test suite can show that they fail on it (mutation testing of the
+// checks; m = 0, the default, is the reference order):
+//   1  the LDF reads the current populations' concentration instead of the previous step's phi
+//   2  the AD lattice is advected by the moments of the populations before NS.step
+//   3  the AD lattice is advected by the moments taken before the PostStreaming BCs
+//   4  the NEQ velocity inlet is applied after streaming (on f_new) instead of before collision
+//   5  the pressure outlet is applied before collision (on f) instead of after streaming
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -68,12 +77,15 @@ struct SysBase {
     virtual void set(int field, const double *in) = 0;
     virtual void outlet(const std::uint8_t *mask, double &avg, double &mn, double &mx) = 0;
     virtual i64 n() const = 0;
+    virtual void set_mutation(int m) = 0;
 };
 
 template <class T> struct System : SysBase {
     orc::Grid g;
     orc_config c;
     std::vector<T> f, fn, gg, gn, phi, S, q, uxa, uya, uza, tau, por, rho_m, ux_m, uy_m, uz_m;
+    std::vector<T> mux, muy, muz;  // moments taken by a planted ordering mistake
+    int mutation = 0;
     std::vector<std::uint8_t> obst, region;
     std::vector<BCHost> nsbc, adbc;
 
@@ -132,13 +144,16 @@ template <class T> struct System : SysBase {
     }
 
     i64 n() const override { return g.n(); }
+    void set_mutation(int m) override { mutation = m; }
 
     void ns_step(int s) {
         for (auto &b : nsbc) {  // PreCollision
-            if (b.d.kind == 1 && !b.bnd.empty())
+            if (b.d.kind == 1 && !b.bnd.empty() && mutation != 4)
                 orc::ns_bc_velocity_neq(f.data(), b.bnd, b.nei, T(b.d.v[0]), T(b.d.v[1]), T(b.d.v[2]));
             else if (b.d.kind == 2)
                 orc::ns_bc_velocity_eq(f.data(), b.mask.data(), g, T(c.rho0), T(b.d.v[0]), T(b.d.v[1]), T(b.d.v[2]));
+            else if (b.d.kind == 3 && mutation == 5)
+                orc::ns_bc_pressure(f.data(), b.mask.data(), g, T(b.d.v[0]));
         }
         const T omega = orc::ns_omega(T(c.tau));
         if (c.ns_op == 0)
@@ -147,9 +162,12 @@ template <class T> struct System : SysBase {
             orc::ns_collide_porous(f.data(), fn.data(), por.empty() ? nullptr : por.data(), g, omega, T(c.rho0));
         else
             orc::ns_collide_pbb(f.data(), fn.data(), por.empty() ? nullptr : por.data(), g, omega, T(c.rho0));
+        if (mutation == 3) moments_of(fn, mux, muy, muz);
         for (auto &b : nsbc) {  // PostStreaming
-            if (b.d.kind == 3) orc::ns_bc_pressure(fn.data(), b.mask.data(), g, T(b.d.v[0]));
+            if (b.d.kind == 3 && mutation != 5) orc::ns_bc_pressure(fn.data(), b.mask.data(), g, T(b.d.v[0]));
             else if (b.d.kind == 4) orc::ns_bc_bounceback(fn.data(), b.mask.data(), g);
+            else if (b.d.kind == 1 && !b.bnd.empty() && mutation == 4)
+                orc::ns_bc_velocity_neq(fn.data(), b.bnd, b.nei, T(b.d.v[0]), T(b.d.v[1]), T(b.d.v[2]));
         }
         std::swap(f, fn);
         (void)s;
@@ -175,22 +193,48 @@ template <class T> struct System : SysBase {
     }
 
     void step(int s) override {
+        if (mutation == 2) moments_of(f, mux, muy, muz);
         ns_step(s);
         if (c.coupling) {  // computeMacroscopicDevice + setAdvectionFromFluidDevice
             ensure_moments();
             uxa = ux_m;
             uya = uy_m;
             uza = uz_m;
+            if (mutation == 2 || mutation == 3) {
+                uxa = mux;
+                uya = muy;
+                uza = muz;
+            }
         }
         if (c.adsorb) {  // zeroSource + computeSource(region)
             std::fill(S.begin(), S.end(), T(0));
+            const T *phi_in = phi.data();
+            std::vector<T> phi_now;
+            if (mutation == 1) {  // the current populations' (clamped) concentration, no lag
+                phi_now.assign(g.n(), T(0));
+                for (i64 i = 0; i < g.n(); ++i) {
+                    T sum = T(0);
+                    for (int k = 0; k < 7; ++k) sum += gg[i * 7 + k];
+                    phi_now[i] = sum < T(c.minC) ? T(c.minC) : sum;
+                }
+                phi_in = phi_now.data();
+            }
             if (c.isotherm == 1)
-                orc::ldf_linear(g.n(), region.data(), phi.data(), S.data(), q.data(), T(c.KHenry), T(c.kA), T(c.minC));
+                orc::ldf_linear(g.n(), region.data(), phi_in, S.data(), q.data(), T(c.KHenry), T(c.kA), T(c.minC));
             else if (c.isotherm == 2)
-                orc::ldf_langmuir(g.n(), region.data(), phi.data(), S.data(), q.data(), T(c.qMax), T(c.bLangmuir),
+                orc::ldf_langmuir(g.n(), region.data(), phi_in, S.data(), q.data(), T(c.qMax), T(c.bLangmuir),
                                   T(c.kLDF), T(c.minC));
         }
         ad_step(s);
+    }
+
+    void moments_of(const std::vector<T> &pop, std::vector<T> &ux, std::vector<T> &uy, std::vector<T> &uz) {
+        const i64 N = g.n();
+        std::vector<T> rho(N);
+        ux.resize(N);
+        uy.resize(N);
+        uz.resize(N);
+        orc::ns_moments(pop.data(), rho.data(), ux.data(), uy.data(), uz.data(), g);
     }
 
     void ensure_moments() {
@@ -288,6 +332,15 @@ void *orc_create(const orc_config *cfg, int prec) {
 }
 
 void orc_destroy(void *h) { delete static_cast<SysBase *>(h); }
+
+// Plants ordering mistake m (1..5, listed at the top of this file) in the system's step sequence;
+// 0 restores the reference order.  For the mutation tests of the sequence pins only.
+int orc_set_step_mutation(void *h, int m) {
+    return guarded([&] {
+        if (m < 0 || m > 5) throw std::runtime_error("orc_set_step_mutation: unknown mutation");
+        static_cast<SysBase *>(h)->set_mutation(m);
+    });
+}
 
 // Runs steps s0 .. s0+nsteps-1.  When `outlet_mask` is non-null, the outlet
 // statistics after every step go to avg/mn/mx (nullable each).  `seconds` (nullable)

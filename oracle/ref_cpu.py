@@ -90,6 +90,7 @@ def lib():
         L.orc_create.restype = vp
         L.orc_create.argtypes = [C.POINTER(_Config), C.c_int]
         L.orc_destroy.argtypes = [vp]
+        L.orc_set_step_mutation.argtypes = [vp, C.c_int]
         L.orc_run.argtypes = [vp, C.c_int, C.c_int, u8p, dp, dp, dp, dp]
         L.orc_get.argtypes = [vp, C.c_int, dp]
         L.orc_set.argtypes = [vp, C.c_int, dp]
@@ -246,6 +247,11 @@ class System:
                              C.byref(sec)))
         self.step_index += nsteps
         return avg, mn, mx, sec.value
+
+    def set_step_mutation(self, m: int) -> None:
+        """Plant ordering mistake m in the step sequence (0: the reference order); for the mutation
+        tests of the sequence pins only (the list is at the top of oracle/ref_cpu.cpp)."""
+        _check(lib().orc_set_step_mutation(self._h, int(m)))
 
     def get(self, fid: int) -> np.ndarray:
         n = self.setup.n * (19 if fid == F else 7 if fid == G else 1)
