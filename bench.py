@@ -1,15 +1,19 @@
 """Benchmark of the fused coupled D3Q19 + D3Q7 step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload C2] [--prec f32]
+    python bench.py [--steps K] [--warmup W] [--workload C3] [--prec f32]
 
-Default workload: C2, 256x128x128 overlapping sphere pack (porosity 0.4),
-Langmuir LDF, 10000 steps.  Prints ONE JSON line with MLUPS (all cells, solids
-included), the HBM roofline of the fused kernel (algorithmic bytes per cell
-update x cells / measured step time, against 8 TB/s), the adjoint/forward time
-ratio of the checkpointed reverse sweep on the differentiation workload (C4,
-128x64x64 partial bounce-back, 4000 steps, checkpoint every 200), the 512^3
-Langmuir pack (C3) timed per storage precision, and the CPU oracle timed
-serially on a bounded sample of the same workload.
+Default workload: C3, the 512^3 overlapping sphere pack (porosity 0.4, 134M
+cells) with the Langmuir LDF, FP32 storage -- the largest single-GPU
+configuration.  Prints ONE JSON line: MLUPS over all cells (solids included),
+the HBM roofline of the fused kernel (algorithmic bytes per cell update x cells
+/ launch time measured with HIP events on the launch stream, against 8 TB/s,
+plus the HBM bytes per launch from the committed rocprofv3 counter summary of
+the same kernel and workload), and the CPU oracle (restatement of the
+reference update) timed serially on a bounded sample of the same workload.
+Extra keys: the same box with FP16 / BF16 storage, the 256x128x128 pack (C2),
+and the adjoint/forward time ratio of the checkpointed gradient on the
+differentiation workload (C4, 128x64x64 partial bounce-back, 4000 steps,
+checkpoint every 200).
 """
 from __future__ import annotations
 
@@ -48,33 +52,45 @@ def cpu_info():
 
 def cpu_baseline(cfg, prec, budget_s):
     """The CPU oracle (restatement of the reference update) on a bounded sample of the workload: serial
-    (the baseline), then the same steps with its per-cell loops on all the threads OpenMP is given."""
+    (the baseline), then the same system with its per-cell loops on all the threads OpenMP is given.
+    The first serial step is timed; more steps are timed only while they fit in `budget_s` (a 512^3
+    step alone takes 15-35 s on one core)."""
     from oracle import ref_cpu as R
     from tests.lattice import oracle_setup
 
     oprec = 8 if prec == "f64" else 4
     model, ncpu = cpu_info()
+    R.lib().orc_set_threads(1)
+    sysm = R.System(oracle_setup(cfg, R), oprec)
     out = []
-    steps = None
-    for threads in (1, 0):  # 0: the OpenMP default (OMP_NUM_THREADS, else every logical CPU)
-        used = int(R.lib().orc_set_threads(threads))
-        sysm = R.System(oracle_setup(cfg, R), oprec)
+    try:
         _, _, _, t1 = sysm.run(1)
-        if steps is None:
+        steps, t = 1, t1
+        if 2 * t1 < budget_s:  # small boxes: the first step was a warm-up, time a window of steps
             steps = int(max(1, min(200, budget_s / max(t1, 1e-6))))
-        _, _, _, t = sysm.run(steps)
+            _, _, _, t = sysm.run(steps)
+        timed = [(1, steps, t)]
+        used = int(R.lib().orc_set_threads(0))  # OpenMP default: OMP_NUM_THREADS, else every logical CPU
+        if used > 1:
+            _, _, _, tp = sysm.run(1)
+            psteps = int(max(1, min(200, budget_s / 2 / max(tp, 1e-6))))
+            _, _, _, tp = sysm.run(psteps)
+            timed.append((used, psteps, tp))
+    finally:
         sysm.close()
+        R.lib().orc_set_threads(0)
+    for used, steps, t in timed:
         mode = "serial" if used == 1 else f"OpenMP, {used} threads"
         out.append({
             "value": round(cfg.n * steps / t / 1e6, 3),
             "unit": "MLUPS",
             "cores": used,
             "kind": "port",
-            "sample": f"{cfg.meta['workload']} {cfg.nx}x{cfg.ny}x{cfg.nz}, {'FP64' if oprec == 8 else 'FP32'} oracle "
-                      f"({mode}), {steps} steps after 1 untimed step ({t:.1f} s), host '{model}' ({ncpu} logical CPUs)",
+            "sample": f"{cfg.meta['workload']} {cfg.nx}x{cfg.ny}x{cfg.nz} (the whole box), {'FP64' if oprec == 8 else 'FP32'} "
+                      f"oracle ({mode}), {steps} step{'s' if steps > 1 else ''} timed ({t:.1f} s), host '{model}' "
+                      f"({ncpu} logical CPUs)",
         })
-    R.lib().orc_set_threads(0)
-    return out[0], out[1]
+    return out[0], (out[1] if len(out) > 1 else None)
 
 
 def stream_probe(gib=2, reps=10):
@@ -167,8 +183,9 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None, ckpt=None, two_
         "forward_ms": round(t_plain, 3),
         "forward_with_checkpoints_ms": round(t_fwd, 3),
         "backward_ms": round(t_bwd, 3),
-        "ratio": round(t_bwd / t_plain, 3),
-        "ratio_incl_checkpointed_forward": round((t_fwd + t_bwd) / t_plain, 3),
+        # the gradient's whole cost (checkpointed forward + reverse sweep) over the plain forward run
+        "ratio": round((t_fwd + t_bwd) / t_plain, 3),
+        "reverse_sweep_ratio": round(t_bwd / t_plain, 3),
     }
     del run
     torch.cuda.empty_cache()
@@ -236,45 +253,54 @@ def adjoint_c5_accuracy(steps=None, k=500):
     }
 
 
-def large_box_timing(workload="C3", precs=("f32", "f16", "bf16"), steps=100, warmup=20):
-    """The 512^3 Langmuir sphere pack (C3) for each storage precision: step time over `steps` launches
-    between two events on the launch stream, MLUPS and the fraction of the algorithmic roofline."""
+def time_steps(sim, steps, warmup):
+    """Warm-up steps, then exactly `steps` launches between two HIP events on the launch stream (with the
+    per-step outlet reduction); returns ms per step."""
     import torch
 
-    from fluidmaxxer_amd import workloads
+    if warmup:
+        sim.run(warmup, log_outlet=True)
+    acc = sim.outlet_acc(steps)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(sim.stream)
+    sim.launch_only(steps, acc=acc)
+    e1.record(sim.stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, acc
+
+
+def storage_timing(cfg, precs, steps=100, warmup=20):
+    """The same workload for each storage precision: step time, MLUPS and the fraction of the
+    algorithmic roofline (121 B/cell for FP16 / BF16 storage, 449 for FP64)."""
+    import torch
+
     from fluidmaxxer_amd.coupled import CoupledSim
 
-    cfg = workloads.build(workload, steps=steps + warmup + 1)
-    out = {"workload": f"{workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, "
+    out = {"workload": f"{cfg.meta['workload']}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, "
                        f"porosity {cfg.meta['porosity']:.4f}, {steps} steps after {warmup}"}
     for prec in precs:
         sim = CoupledSim(cfg, prec=prec)
-        sim.run(warmup, log_outlet=True)
-        acc = sim.outlet_acc(steps)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(sim.stream)
-        sim.launch_only(steps, acc=acc)
-        e1.record(sim.stream)
-        torch.cuda.synchronize()
-        ms_step = e0.elapsed_time(e1) / steps
-        achieved = B_ALG[prec] * cfg.n / (ms_step / 1e3) / 1e9
+        ms_step, acc = time_steps(sim, steps, warmup)
+        b_alg = B_ALG[prec] + (B_FIELD[prec] if cfg.field is not None else 0)
+        achieved = b_alg * cfg.n / (ms_step / 1e3) / 1e9
         out[prec] = {"mlups": round(cfg.n / (ms_step / 1e3) / 1e6, 1), "ms_per_step": round(ms_step, 4),
                      "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
         del sim, acc
         torch.cuda.empty_cache()
-    if "f32" in out and "f16" in out:
-        out["f16_over_f32"] = round(out["f16"]["mlups"] / out["f32"]["mlups"], 3)
     return out
 
 
 def traffic_from_profile(workload, prec):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload.lower()}_{prec}.json")
+    """HBM bytes per launch of the fused kernel from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs, scaled by a plain copy's reading) of this workload and storage, and its path."""
+    rel = os.path.join("profiles", f"traffic_{workload.lower()}_{prec}.json")
+    path = os.path.join(ROOT, rel)
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as fh:
         d = json.load(fh)
-    return d.get("hbm_bytes_per_launch")
+    return d.get("hbm_bytes_per_launch"), rel
 
 
 def main():
@@ -282,10 +308,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="C2")
-    ap.add_argument("--dims", default=None, help="override box, e.g. 512x512x512")
+    ap.add_argument("--workload", default="C3")
+    ap.add_argument("--dims", default=None, help="override box, e.g. 256x256x256")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64", "f16", "bf16"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="CPU baseline budget (one serial step is always timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-outlet", action="store_true", help="skip the per-step outlet reduction")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -294,7 +321,9 @@ def main():
                     help="instead of the benchmark, measure plain streaming bandwidth (copy / read / write)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
-    ap.add_argument("--no-large-box", action="store_true", help="skip the 512^3 (C3) timing per storage precision")
+    ap.add_argument("--no-storage-sweep", "--no-large-box", dest="no_storage_sweep", action="store_true",
+                    help="skip the FP16 / BF16 storage timings of the same workload")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 256x128x128 (C2) FP32 timing")
     ap.add_argument("--adjoint-steps", type=int, default=None, metavar="N",
                     help="steps of the adjoint timing (default: the workload's own)")
     ap.add_argument("--adjoint-c5-accuracy", action="store_true",
@@ -346,7 +375,7 @@ def main():
 
     dims = tuple(int(v) for v in args.dims.split("x")) if args.dims else None
     spec_steps = workloads.SPECS[args.workload]["steps"]
-    steps = args.steps if args.steps is not None else spec_steps
+    steps = args.steps if args.steps is not None else min(spec_steps, 200)
     cfg = workloads.build(args.workload, dims=dims, steps=max(steps + args.warmup + 1, spec_steps))
     n = cfg.n
     sim = CoupledSim(cfg, prec=args.prec)
@@ -370,15 +399,13 @@ def main():
     achieved = b_alg * n / (ms_step / 1e3) / 1e9
     c_out_last = None
     if log:
-        import ctypes
-
         from fluidmaxxer_amd import abi
         mean = torch.empty(steps, dtype=torch.float64, device=sim.device)
         abi.call("fmx_outlet_mean", acc.data_ptr(), steps, cfg.ny * cfg.nz, mean.data_ptr(), sim.stream.cuda_stream)
         torch.cuda.synchronize()
         c_out_last = float(mean[-1].item())
 
-    traffic = traffic_from_profile(args.workload, args.prec) if dims is None else None
+    traffic, traffic_src = traffic_from_profile(args.workload, args.prec) if dims is None else (None, None)
     line = {
         "metric": "Coupled D3Q19+D3Q7 MLUPS & %HBM roofline; adjoint/forward time ratio",
         "value": round(mlups, 1),
@@ -411,18 +438,29 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": (f"{traffic_src}: rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this kernel and workload, "
+                               "scaled by a plain copy's reading; not measured in this run") if traffic_src else None,
             "bytes_per_cell_update": b_alg,
             "kernel": "k_coupled (one launch per step)",
         },
     }
-    del sim
+    del sim, acc
     torch.cuda.empty_cache()
+    if not args.no_storage_sweep and args.prec == "f32":
+        st = storage_timing(cfg, ("f16", "bf16"), steps=steps, warmup=min(args.warmup, 20))
+        st["f16_over_f32"] = round(st["f16"]["mlups"] / mlups, 3)
+        st["bf16_over_f32"] = round(st["bf16"]["mlups"] / mlups, 3)
+        line["storage"] = st
+    if not args.no_c2 and args.workload != "C2" and dims is None:
+        c2 = workloads.build("C2", steps=max(steps, 100) + 21)
+        line["c2"] = storage_timing(c2, ("f32",), steps=max(steps, 100), warmup=20)
+        del c2
     if not args.no_adjoint and args.prec in ("f32", "f64"):
         line["adjoint"] = adjoint_timing(prec=args.prec)
-    if not args.no_large_box and args.workload != "C3" and dims is None:
-        line["large_box"] = large_box_timing()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"], line["cpu_baseline_all_cores"] = cpu_baseline(cfg, args.prec, args.cpu_seconds)
+        line["cpu_baseline"], all_cores = cpu_baseline(cfg, args.prec, args.cpu_seconds)
+        if all_cores is not None:
+            line["cpu_baseline_all_cores"] = all_cores
     print(json.dumps(line), flush=True)
 
 

@@ -13,9 +13,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_bench_line_contract():
-    print("bench.py --steps 20 --warmup 5 (short line)", flush=True)
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "20", "--warmup", "5",
-                        "--no-large-box", "--no-adjoint", "--cpu-seconds", "0.5"],
+    print("bench.py --workload C2 --steps 20 --warmup 5 (short line)", flush=True)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "C2", "--steps", "20",
+                        "--warmup", "5", "--no-storage-sweep", "--no-adjoint", "--cpu-seconds", "0.5"],
                        capture_output=True, text=True, timeout=300, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [ln for ln in r.stdout.strip().splitlines() if ln.startswith("{")]
@@ -31,5 +31,6 @@ def test_bench_line_contract():
     rf = d["roofline"]
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
     assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3 and 0 < rf["frac"] < 1
+    assert rf["traffic"] > 0 and rf["traffic_source"].startswith("profiles/traffic_c2_f32.json")
     cb = d["cpu_baseline"]
     assert cb["cores"] == 1 and cb["kind"] in ("port", "reference") and cb["value"] > 0 and cb["sample"]
