@@ -21,6 +21,7 @@ NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
+OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
 
@@ -56,7 +57,7 @@ class CoupledBufs(C.Structure):
         ("q", C.c_void_p), ("phi", C.c_void_p * 2),
         ("links", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
         ("inlet_raw", C.c_void_p), ("inlet_series", C.c_void_p), ("outlet_acc", C.c_void_p),
-        ("outlet_ring", C.c_void_p),
+        ("outlet_ring", C.c_void_p), ("inlet_series_len", C.c_int64), ("outlet_acc_rows", C.c_int64),
     ]
 
 
@@ -65,6 +66,7 @@ class AdjointBufs(C.Structure):
         ("lam_f", C.c_void_p * 2), ("lam_g", C.c_void_p * 2),
         ("lam_q", C.c_void_p), ("lam_phi", C.c_void_p),
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("outlet_weight", C.c_void_p),
+        ("outlet_weight_len", C.c_int64),
         ("grad_iso", C.c_void_p), ("grad_field", C.c_void_p), ("grad_series", C.c_void_p),
     ]
 

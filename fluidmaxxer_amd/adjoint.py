@@ -79,6 +79,8 @@ class AdjointRun:
                             if self.sim.inlet_series is not None else None)
         self.acc = None
         self.forward_done = False
+        # counts forward() calls: a reverse sweep belongs to the forward run that wrote the checkpoints
+        self.generation = 0
 
     # ------------------------------------------------------------ parameters
     def set_params(self, iso_a=None, iso_b=None, field=None, series=None):
@@ -111,18 +113,25 @@ class AdjointRun:
                  self.ckpt.data_ptr(), self.ckpt_prec, sim._s())
         sim.cur = self.nsteps & 1
         sim.step_index = self.nsteps
+        self.generation += 1
         mean = torch.empty(self.nsteps, dtype=torch.float64, device=self.device)
         abi.call("fmx_outlet_mean", self.acc.data_ptr(), self.nsteps, self.cfg.ny * self.cfg.nz, mean.data_ptr(),
                  sim._s())
         self.forward_done = True
         return mean
 
-    def backward(self, weight):
+    def backward(self, weight, generation=None):
         """Gradients of a loss with dL/dC_out(s) = weight[s-1]: dict of iso_a, iso_b (floats),
-        field (per cell, or None) and series (indexed like inlet_series, or None)."""
+        field (per cell, or None) and series (indexed like inlet_series, or None).  `generation`, when
+        given, is the value of self.generation right after the forward run these gradients belong to:
+        a later forward() has overwritten that run's checkpoints and parameters, which is an error."""
         torch = _torch()
         if not self.forward_done:
             raise abi.FmxError("backward() needs a forward() with the current parameters")
+        if generation is not None and generation != self.generation:
+            raise abi.FmxError(f"this reverse sweep belongs to forward run {generation}, but the AdjointRun has "
+                               f"since run forward {self.generation - generation} more time(s) and holds that run's "
+                               f"checkpoints; use one AdjointRun per node of the autograd graph")
         w = torch.as_tensor(weight).to(self.device, torch.float64).contiguous()
         if w.numel() != self.nsteps:
             raise abi.FmxError(f"weight must have {self.nsteps} entries")
@@ -138,6 +147,7 @@ class AdjointRun:
             self.workspace = torch.empty(max(wb, 1), dtype=torch.uint8, device=self.device)
         a.workspace, a.workspace_bytes = self.workspace.data_ptr(), self.workspace.numel()
         a.outlet_weight = w.data_ptr()
+        a.outlet_weight_len = w.numel()
         a.grad_iso = self.grad_iso.data_ptr()
         a.grad_field = self.grad_field.data_ptr() if self.grad_field is not None else None
         a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
@@ -171,12 +181,14 @@ def _autograd_function():
             ctx.run = run
             ctx.meta = (iso_a.dtype, iso_a.device, None if field is None else (field.dtype, field.device),
                         None if series is None else (series.dtype, series.device))
-            return run.forward()
+            out = run.forward()
+            ctx.generation = run.generation
+            return out
 
         @staticmethod
         def backward(ctx, grad_c):
             run = ctx.run
-            g = run.backward(grad_c.detach().to(torch.float64))
+            g = run.backward(grad_c.detach().to(torch.float64), generation=ctx.generation)
             adt, adev, fmeta, smeta = ctx.meta
             ga = torch.tensor(g["iso_a"], dtype=adt, device=adev)
             gb = torch.tensor(g["iso_b"], dtype=adt, device=adev)

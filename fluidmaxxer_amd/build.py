@@ -101,6 +101,13 @@ def build_cpp_driver() -> list:
         src = os.path.join(ROOT, "tests", "cpp", f"{name}.cpp")
         for dt in ("float", "double"):
             outs.append(_gxx(src, os.path.join(out_dir, f"{name}_{dt}"), dt))
+    # the device FP16 decode, compiled for the host (no library needed)
+    exe = os.path.join(out_dir, "half_units")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{CSRC}", os.path.join(ROOT, "tests", "cpp", "half_units.cpp"),
+                        "-o", exe], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for half_units.cpp:\n{r.stderr}")
+    outs.append(exe)
     return outs
 
 

@@ -156,7 +156,9 @@ class CoupledSim:
         b.tau_field = self.tau_field.data_ptr() if self.tau_field is not None else None
         b.inlet_raw = self.inlet_raw.data_ptr() if self.inlet_raw is not None else None
         b.inlet_series = self.inlet_series.data_ptr() if self.inlet_series is not None else None
+        b.inlet_series_len = self.inlet_series.numel() if self.inlet_series is not None else 0
         b.outlet_acc = acc.data_ptr() if acc is not None else None
+        b.outlet_acc_rows = acc.numel() // (abi.OUTLET_SLOTS * abi.OUTLET_STRIDE) if acc is not None else 0
         if acc is not None:  # outlet sums in a fixed order (repeatable to the bit)
             if self._ring is None:
                 self._ring = _torch().empty(abi.OUTLET_RING_ROWS * self.cfg.ny * self.cfg.nz,

@@ -731,7 +731,7 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
 static bool ckpt_prec_ok(int prec, int cp) { return cp == prec || (prec == FMX_F32 && (cp == FMX_F16 || cp == FMX_BF16)); }
 
 static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
-                         int32_t k, int cp, const char *who) {
+                         int32_t k, int cp, const char *who, bool writes_outlet = false) {
     if (int rc = fmx_coupled_validate(d, b)) return rc;
     if (!ckpt_prec_ok(d->prec, cp))
         return set_error(FMX_ERR_UNSUPPORTED,
@@ -743,6 +743,14 @@ static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
         return set_error(FMX_ERR_UNSUPPORTED, "%s: the NEQ velocity inlet has no adjoint (use the equilibrium inlet)",
                          who);
     if (step0 < 0 || nsteps < 1 || k < 1) return set_error(FMX_ERR_ARG, "%s: bad step range or interval", who);
+    return check_run_buffers(*d, *b, step0, nsteps, who, writes_outlet);
+}
+
+// The loss weights a reverse sweep reads: outlet_weight[0 .. need-1].
+static int check_weights(const fmx_adjoint_bufs *adj, i64 need, const char *who) {
+    if (adj && adj->outlet_weight && adj->outlet_weight_len < need)
+        return set_error(FMX_ERR_ARG, "%s: outlet_weight has %lld entries, %lld needed", who,
+                         (long long)adj->outlet_weight_len, (long long)need);
     return FMX_OK;
 }
 
@@ -796,15 +804,14 @@ static int backward(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 st
                  {adj.grad_series ? adj.grad_series + step0 : nullptr, size_t(nsteps) * sizeof(double)}};
     for (const Z &zz : zeros)
         if (zz.p && zz.bytes)
-            if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_backward: clear")) return rc;
+            if (int rc = fmx_fill_zero(zz.p, zz.bytes, s)) return rc;  // a kernel: safe under graph capture
     // the workspace: per-block isotherm sums, then the per-block series sums of up to kSeriesRows
     // reverse steps
     const int nblocks = int(coupled_blocks(prm.gx, prm.gy, prm.gz));
     const bool want_series = d.ad_inlet == 2 && adj.grad_series;
     double *iso_part = static_cast<double *>(adj.workspace);
     double *ser_rows = iso_part + 2 * i64(nblocks);
-    if (int rc = check_hip(hipMemsetAsync(iso_part, 0, 2 * size_t(nblocks) * sizeof(double), s),
-                           "fmx_adjoint_backward: clear"))
+    if (int rc = fmx_fill_zero(iso_part, 2 * size_t(nblocks) * sizeof(double), s))
         return rc;
     int rows = 0;  // rows filled; row r holds reverse step top - r
     i64 top = 0;
@@ -898,7 +905,7 @@ static int single_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int
                      {iso_part, 2 * size_t(nblocks) * sizeof(double)}};
         for (const Z &zz : zeros)
             if (zz.p && zz.bytes)
-                if (int rc = check_hip(hipMemsetAsync(zz.p, 0, zz.bytes, s), "fmx_adjoint_step: clear")) return rc;
+                if (int rc = fmx_fill_zero(zz.p, zz.bytes, s)) return rc;
     }
     const StateView<P> st{static_cast<S *>(b.f[cur]), static_cast<S *>(b.g[cur]), static_cast<C *>(b.q),
                           static_cast<C *>(b.phi[cur])};
@@ -945,7 +952,7 @@ int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *d, int ckpt_prec) {
 int fmx_adjoint_forward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
                         int32_t k, int cur, void *ckpt, int ckpt_prec, void *stream) {
     FMX_TRY_BEGIN
-    if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_forward")) return rc;
+    if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_forward", true)) return rc;
     if ((cur != 0 && cur != 1) || !ckpt) return set_error(FMX_ERR_ARG, "fmx_adjoint_forward: bad cur or checkpoint store");
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
@@ -1030,6 +1037,7 @@ int fmx_adjoint_step(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int c
                          (long long)(adj->workspace ? adj->workspace_bytes : 0), (long long)wb);
     if (d->ad_inlet != 2 && adj->grad_series)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_step: grad_series needs the per-step inlet series (ad_inlet 2)");
+    if (int rc = check_weights(adj, step + 1, "fmx_adjoint_step")) return rc;
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
         case FMX_F32: return single_step<FMX_F32>(*d, *b, cur, step, lam_cur, first != 0, *adj, s);
@@ -1063,6 +1071,7 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
                          (long long)(adj->workspace ? adj->workspace_bytes : 0), (long long)wb);
     if (d->ad_inlet != 2 && adj->grad_series)
         return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: grad_series needs the per-step inlet series (ad_inlet 2)");
+    if (int rc = check_weights(adj, nsteps, "fmx_adjoint_backward")) return rc;
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
     switch (d->prec) {

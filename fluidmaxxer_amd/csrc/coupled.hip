@@ -384,6 +384,20 @@ static int view(const fmx_coupled_desc &d, const void *pop, const void *inlet_ra
 }
 
 
+// The lengths a run over steps step0 .. step0+nsteps-1 reads and writes: the inlet series (entries step0 ..
+// step0+nsteps-1) and the outlet accumulator (rows 0 .. nsteps-1).
+int check_run_buffers(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, i64 nsteps, const char *who,
+                      bool writes_outlet) {
+    if (d.ad_inlet == 2 && b.inlet_series_len < step0 + nsteps)
+        return set_error(FMX_ERR_ARG, "%s: steps %lld .. %lld read inlet_series[%lld], but inlet_series_len is %lld", who,
+                         (long long)step0, (long long)(step0 + nsteps - 1), (long long)(step0 + nsteps - 1),
+                         (long long)b.inlet_series_len);
+    if (writes_outlet && b.outlet_acc && b.outlet_acc_rows < nsteps)
+        return set_error(FMX_ERR_ARG, "%s: %lld steps write %lld outlet_acc rows, but outlet_acc_rows is %lld", who,
+                         (long long)nsteps, (long long)nsteps, (long long)b.outlet_acc_rows);
+    return FMX_OK;
+}
+
 }  // namespace fmx
 
 using namespace fmx;
@@ -483,6 +497,7 @@ int fmx_coupled_run(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_
     if (rc) return rc;
     if (cur != 0 && cur != 1) return set_error(FMX_ERR_ARG, "fmx_coupled_run: cur must be 0 or 1");
     if (nsteps < 0 || step0 < 0) return set_error(FMX_ERR_ARG, "fmx_coupled_run: bad step range");
+    if (int rc2 = check_run_buffers(*d, *b, step0, nsteps, "fmx_coupled_run")) return rc2;
     hipStream_t s = as_stream(stream);
     switch (d->prec) {
         case FMX_F32: return run_coupled_steps<FMX_F32>(*d, *b, step0, nsteps, cur, s);

@@ -6,6 +6,7 @@
 #include <cstdlib>
 
 #include "fmx_exact.hpp"
+#include "fmx_half.hpp"
 #include "fmx_internal.hpp"
 
 namespace fmx {
@@ -96,16 +97,13 @@ template <class T, class O> __device__ __forceinline__ void st_plane_nt(T *base,
     }
 }
 
-// A stored 16-bit value as float.  FP16 is widened by moving its exponent and mantissa bits into
-// an FP32 pattern and scaling by 2^112 (exact for normal and subnormal halves; FP32 subnormals are
-// kept in this build), which runs faster in this kernel on gfx950 than the hardware conversion or a
-// shorter sign-extending form (measured); BF16 is the top half of an FP32.
+// A stored 16-bit value as float: FP16 through half_bits_to_float (fmx_half.hpp), BF16 is the top half
+// of an FP32.
 template <class S> __device__ __forceinline__ float widen(S v) {
     if constexpr (std::is_same_v<S, _Float16>) {
         unsigned short h;
         __builtin_memcpy(&h, &v, 2);
-        const float mag = __uint_as_float((unsigned(h) & 0x7fffu) << 13) * 0x1p112f;
-        return __uint_as_float(__float_as_uint(mag) | ((unsigned(h) & 0x8000u) << 16));
+        return half_bits_to_float(h);
     } else {
         return float(v);
     }
@@ -254,6 +252,10 @@ int launch_coupled_pair(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParam
 template <int P>
 int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
                       hipStream_t s, i64 acc_row0 = 0);
+// FMX_ERR_ARG unless the inlet series (and, when the run writes it, the outlet accumulator) of b cover
+// steps step0 .. step0+nsteps-1.
+int check_run_buffers(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, i64 nsteps, const char *who,
+                      bool writes_outlet = true);
 // Clears the frozen-solid bits of the link words (every cell back on the full update path).
 int clear_frozen(const fmx_coupled_desc &d, std::uint32_t *links, hipStream_t s);
 

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256)
     if (phi_out) phi_out[cell] = phi;
 }
 
-// k_ad_dirichlet (ADDirichlet.cu:11-30), k_ad_dirichlet_tanh_pulse_d3q7 (ADDirichletTanhPulse.cu:66-86)
+// k_ad_dirichlet (ADDirichlet.cu:11-30), k_ad_dirichlet_tanh_pulse_d3q7 (ADDirichletTanhPulse.cu:11-31, launched at :91-96)
 template <class T>
 __global__ void __launch_bounds__(256) k_ad_bc_dirichlet(T *__restrict__ g, const i64 *__restrict__ cells, i64 count,
                                                          i64 n, T phi) {

@@ -1,5 +1,5 @@
 #pragma once
-// Smooth tanh inlet pulse (ADDirichletTanhPulse.hpp:12-38, ADDirichletTanhPulse.cu:11-157 of
+// Smooth tanh inlet pulse (ADDirichletTanhPulse.hpp:12-38, ADDirichletTanhPulse.cu:11-100 of
 // the reference), PreCollision on g: g <- phi_in(step) * w on the mask with
 //   phi_in = base + (DATA_TYPE) (0.5 (tanh((s - t_on)/k) - tanh((s - t_off)/k))) * amp,
 // the pulse evaluated in double on the host.

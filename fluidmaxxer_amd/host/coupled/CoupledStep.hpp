@@ -90,11 +90,11 @@ public:
             }
             return;
         }
+        if (series_) ensureSeries(s0 + n);  // before the prologue: a per-step inlet needs its series buffer
         if (!primed_) {
             fmx::check(fmx_coupled_prologue(&desc_, bufsFor(nullptr), 0, nullptr), "fmx_coupled_prologue");
             primed_ = true;
         }
-        if (series_) ensureSeries(s0 + n);
         double *acc = nullptr;
         if (logOutlet) acc = accFor(n);
         fmx::check(fmx_coupled_run(&desc_, bufsFor(acc), s0, n, 0, nullptr), "fmx_coupled_run");
@@ -254,7 +254,9 @@ private:
         bufs_.tau_field = tau_field_;
         bufs_.inlet_raw = d_inlet_raw_;
         bufs_.inlet_series = d_series_;
+        bufs_.inlet_series_len = d_series_ ? series_cap_ : 0;
         bufs_.outlet_acc = acc;
+        bufs_.outlet_acc_rows = acc ? acc_cap_ : 0;
         if (acc) {  // breakthrough sums in a fixed order (the same curve on every run)
             if (!d_ring_)
                 d_ring_ = static_cast<double *>(fmx::mallocBytes(sizeof(double) * FMX_OUTLET_RING_ROWS * nyz_));

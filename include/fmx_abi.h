@@ -26,7 +26,9 @@
 extern "C" {
 #endif
 
-#define FMX_ABI_VERSION 1
+/* 2: buffer lengths in fmx_coupled_bufs (inlet_series_len, outlet_acc_rows) and fmx_adjoint_bufs
+ * (outlet_weight_len), checked before any launch */
+#define FMX_ABI_VERSION 2
 
 /* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
 enum { FMX_F32 = 0, FMX_F64 = 1, FMX_F16 = 2, FMX_BF16 = 3 };
@@ -125,10 +127,10 @@ int fmx_ad_collide_stream(int prec, const void *g, void *g_new, void *phi_out, c
                           const void *uy, const void *uz, const void *S, const fmx_grid *grid, double min_conc,
                           void *stream);
 /* ADDirichlet (post, on g_new: ADDirichlet.cu:11-53) and ADDirichletTanhPulse (pre, on g:
- * ADDirichletTanhPulse.cu:66-155): g <- phi * w on the listed cells */
+ * ADDirichletTanhPulse.cu:66-100): g <- phi * w on the listed cells */
 int fmx_ad_bc_dirichlet(int prec, void *g, const int64_t *cells, int64_t count, double phi, int64_t ncells,
                         void *stream);
-/* ADDirichletTanhPulse (pre, on g: ADDirichletTanhPulse.cu:66-155) with the per-step inlet value on
+/* ADDirichletTanhPulse (pre, on g: ADDirichletTanhPulse.cu:66-100) with the per-step inlet value on
  * the device: g <- T(series[step]) * w on the listed cells (the pulse, or any inlet concentration
  * curve, tabulated per step; the host computes nothing per step) */
 int fmx_ad_bc_dirichlet_series(int prec, void *g, const int64_t *cells, int64_t count, const double *series,
@@ -197,6 +199,9 @@ typedef struct fmx_coupled_bufs {
                                  cells store phi there and every step's sum is added in a fixed order (every
                                  FMX_OUTLET_RING_ROWS steps and at the end of a run), so the breakthrough
                                  curve repeats to the bit; without it the sums are FP64 atomic adds */
+    int64_t inlet_series_len; /* entries of inlet_series: a run over steps step0 .. step0+nsteps-1 reads
+                                 entries step0 .. step0+nsteps-1, so it needs step0 + nsteps of them */
+    int64_t outlet_acc_rows;  /* rows of outlet_acc: a run of nsteps steps writes rows 0 .. nsteps-1 */
 } fmx_coupled_bufs;
 #define FMX_OUTLET_RING_ROWS 64
 
@@ -263,6 +268,7 @@ typedef struct fmx_adjoint_bufs {
     void *workspace;             /* fmx_adjoint_workspace_bytes(desc) bytes (scratch) */
     int64_t workspace_bytes;     /* size of workspace */
     const double *outlet_weight; /* nsteps: dL/dC_out(step0 + i); NULL = no loss (all gradients 0) */
+    int64_t outlet_weight_len;   /* entries of outlet_weight (>= nsteps; fmx_adjoint_step: > step) */
     double *grad_iso;            /* 2 doubles (written) */
     double *grad_field;          /* N doubles (written) or NULL */
     double *grad_series;         /* indexed like inlet_series; entries step0 .. step0+nsteps-1 written, or NULL */

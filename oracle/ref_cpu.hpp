@@ -473,7 +473,7 @@ void ad_collide(const T *g, T *g_new, T *phi_out, const T *tau, const T *ux, con
 }
 
 // k_ad_dirichlet (ADDirichlet.cu:11-30) and k_ad_dirichlet_tanh_pulse_d3q7
-// (ADDirichletTanhPulse.cu:66-86): g <- phi * w on the mask.
+// (ADDirichletTanhPulse.cu:11-31): g <- phi * w on the mask.
 template <class T> void ad_bc_dirichlet(T *g, const std::uint8_t *mask, const Grid &gr, T phi) {
     ORC_PAR_FOR
     for (i64 c = 0; c < gr.n(); ++c) {
@@ -482,7 +482,7 @@ template <class T> void ad_bc_dirichlet(T *g, const std::uint8_t *mask, const Gr
     }
 }
 
-// Tanh pulse inlet value (ADDirichletTanhPulse.cu:127-138), host double arithmetic.
+// Tanh pulse inlet value (ADDirichletTanhPulse.cu:72-83), host double arithmetic.
 template <class T> T tanh_pulse_value(int step, T base, T amp, int t_on, int t_off, T k) {
     const double tt = double(step), ton = double(t_on), toff = double(t_off), kk = double(k);
     const double pulse = 0.5 * (std::tanh((tt - ton) / kk) - std::tanh((tt - toff) / kk));

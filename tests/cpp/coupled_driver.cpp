@@ -6,6 +6,8 @@
 //
 //   coupled_driver <solid.vti> <region.vti> <out_dir> <steps> <fused|operators> key=value...
 //   keys: tau u_in tau_ad K kA qmax b kldf inlet=eq|neq iso=linear|langmuir
+//         adinlet=dirichlet|pulse (ADDirichlet C = 1, or ADDirichletTanhPulse with
+//         pulse=base,amp,t_on,t_off,k), split=N (fused: run steps 1..N, then the rest, in two calls)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -27,7 +29,8 @@ int main(int argc, char **argv) {
     }
     std::map<std::string, std::string> kv = {{"tau", "0.8"}, {"u_in", "0.05"}, {"tau_ad", "0.7"}, {"K", "0.5"},
                                              {"kA", "0.05"}, {"qmax", "2"},    {"b", "5"},        {"kldf", "0.02"},
-                                             {"inlet", "eq"}, {"iso", "linear"}};
+                                             {"inlet", "eq"}, {"iso", "linear"}, {"adinlet", "dirichlet"},
+                                             {"pulse", "0.1,0.9,20,70,6"}, {"split", "0"}};
     for (int i = 6; i < argc; ++i) {
         std::string a = argv[i];
         const auto eq = a.find('=');
@@ -68,7 +71,17 @@ int main(int argc, char **argv) {
     else ns.addBoundary<VelocityDirichletEq<D3Q19>>(p, inlet, num("u_in"), DATA_TYPE(0), DATA_TYPE(0));
     ns.addBoundary<PressureDirichlet<D3Q19>>(p, outlet, p.rho0);
     ad.setTauField(std::vector<DATA_TYPE>(geom.mask.size(), p.tau_ad));
-    ad.addBoundary<ADDirichlet<D3Q7>>(p, inlet, DATA_TYPE(1.0));
+    if (kv["adinlet"] == "pulse") {
+        double base = 0, amp = 0, k = 1;
+        int t_on = 0, t_off = 0;
+        if (std::sscanf(kv["pulse"].c_str(), "%lf,%lf,%d,%d,%lf", &base, &amp, &t_on, &t_off, &k) != 5) {
+            std::fprintf(stderr, "pulse=base,amp,t_on,t_off,k expected\n");
+            return 2;
+        }
+        ad.addBoundary<ADDirichletTanhPulse<D3Q7>>(p, inlet, DATA_TYPE(base), DATA_TYPE(amp), t_on, t_off, DATA_TYPE(k));
+    } else {
+        ad.addBoundary<ADDirichlet<D3Q7>>(p, inlet, DATA_TYPE(1.0));
+    }
     ad.addBoundary<ADNeumannOutlet<D3Q7>>(p, outlet);
     AdsorbedField q(p);
     LinearLDF<D3Q7> lin(p);
@@ -84,7 +97,13 @@ int main(int argc, char **argv) {
         else cs.reset(new CoupledStep<D3Q19, D3Q7>(ns, ad, q, lin, reg.mask));
         wasFused = cs->fused();
         if (!wasFused) std::fprintf(stderr, "not fused: %s\n", cs->whyNotFused());
-        cs->run(1, steps, true);
+        const int split = std::atoi(kv["split"].c_str());
+        if (split > 0 && split < steps) {  // two calls: the second grows the cached per-step inlet values
+            cs->run(1, split, true);
+            cs->run(split + 1, steps - split, true);
+        } else {
+            cs->run(1, steps, true);
+        }
         cs->finish();
         c_out = cs->outletMeans();
     } else {

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "boundary/ad/ADDirichletTanhPulse.hpp"
 #include "core/descriptors/D3Q7.hpp"
 #include "core/io/ADUnitConverter.hpp"
 #include "core/io/Correlations.hpp"
@@ -45,6 +46,18 @@ int main(int argc, char **argv) {
                 ad.tauFromDiffusivityPhys(Deff), double(tau[0]), double(tau[1]), uc.lengthPhysToLattice(1e-3),
                 uc.timeLatticeToPhys(1000.0), uc.viscosityPhysToLattice(1e-6), uc.densityLatticeToPhys(1.02),
                 uc.velocityLatticeToPhys(0.01));
+
+    // ADDirichletTanhPulse::valueAt across both edges of a pulse (base 0.1, amp 0.9, t_on 300, t_off 900, k 40)
+    LBMParams pp{};
+    pp.Nx = 5;
+    pp.Ny = 4;
+    pp.Nz = 3;
+    const ADDirichletTanhPulse<D3Q7> pulse(pp, std::vector<std::uint8_t>(60, 0), DATA_TYPE(0.1), DATA_TYPE(0.9), 300, 900,
+                                           DATA_TYPE(40));
+    std::printf("{\"pulse\": [");
+    const int pulse_steps[] = {0, 1, 150, 260, 299, 300, 301, 340, 600, 880, 900, 901, 950, 1200, 5000};
+    for (int i = 0; i < 15; ++i) std::printf("%s[%d, %.17g]", i ? ", " : "", pulse_steps[i], double(pulse.valueAt(pulse_steps[i])));
+    std::printf("]}\n");
 
     std::vector<fmx_vti::Field> fields;
     fields.push_back({"mask", 1, [&](std::int64_t c, int) { return DATA_TYPE(mask[std::size_t(c)]); }});

@@ -20,7 +20,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(names) == set(abi._SIGS), set(names) ^ set(abi._SIGS)
-    assert L.fmx_abi_version() == 1
+    assert L.fmx_abi_version() == 2
 
 
 def test_validate_rejects_bad_descriptors():
@@ -44,6 +44,55 @@ def test_validate_rejects_bad_descriptors():
     assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 1
     with pytest.raises(abi.FmxError):
         abi.call("fmx_coupled_run", C.byref(d), C.byref(b), 1, 1, 0, None)
+
+
+def test_short_buffers_are_refused_before_any_launch():
+    """ABI 2 carries the lengths of the buffers a run indexes by step: a run over steps step0 ..
+    step0+nsteps-1 needs step0+nsteps inlet-series entries and nsteps outlet rows, a reverse sweep
+    nsteps loss weights (fmx_adjoint_step: step+1).  A short buffer is FMX_ERR_ARG, returned before
+    anything is launched (these calls run without a GPU)."""
+    n = 8 * 4 * 4
+    cfg = CoupledConfig(8, 4, 4, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8), ad_inlet="series")
+    L = abi.lib()
+    d = cfg.desc(abi.F32)
+    assert d.ad_inlet == 2
+    b = abi.CoupledBufs()
+    for i in range(2):
+        b.f[i] = b.g[i] = b.phi[i] = 0x1000
+    b.q = b.links = b.inlet_raw = b.inlet_series = 0x1000
+    b.inlet_series_len = 10
+    # steps 5 .. 14 read inlet_series[14]: 15 entries needed
+    assert L.fmx_coupled_run(C.byref(d), C.byref(b), 5, 10, 0, None) == abi.ERR_ARG
+    assert b"inlet_series_len is 10" in L.fmx_last_error()
+    b.inlet_series_len = 15
+    b.outlet_acc = 0x2000
+    b.outlet_acc_rows = 9
+    assert L.fmx_coupled_run(C.byref(d), C.byref(b), 5, 10, 0, None) == abi.ERR_ARG
+    assert b"outlet_acc_rows is 9" in L.fmx_last_error()
+    assert L.fmx_coupled_step(C.byref(d), C.byref(b), 15, 0, None) == abi.ERR_ARG  # reads inlet_series[15]
+    rc = L.fmx_adjoint_forward(C.byref(d), C.byref(b), 5, 10, 4, 0, 0x3000, abi.F32, None)
+    assert rc == abi.ERR_ARG and b"outlet_acc_rows" in L.fmx_last_error()
+    b.outlet_acc_rows = 10
+    b.inlet_series_len = 14
+    rc = L.fmx_adjoint_forward(C.byref(d), C.byref(b), 5, 10, 4, 0, 0x3000, abi.F32, None)
+    assert rc == abi.ERR_ARG and b"inlet_series" in L.fmx_last_error()
+    b.inlet_series_len = 15
+    a = abi.AdjointBufs()
+    for i in range(2):
+        a.lam_f[i] = a.lam_g[i] = 0x1000
+    a.lam_q = a.lam_phi = a.grad_iso = a.workspace = 0x1000
+    a.workspace_bytes = 1 << 40
+    a.outlet_weight, a.outlet_weight_len = 0x4000, 9
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 5, 10, 4, 0x3000, abi.F32, 0x5000, 4, C.byref(a), None)
+    assert rc == abi.ERR_ARG and b"outlet_weight has 9 entries, 10 needed" in L.fmx_last_error()
+    a.outlet_weight_len = 10
+    b.inlet_series_len = 14
+    rc = L.fmx_adjoint_backward(C.byref(d), C.byref(b), 5, 10, 4, 0x3000, abi.F32, 0x5000, 4, C.byref(a), None)
+    assert rc == abi.ERR_ARG and b"inlet_series" in L.fmx_last_error()
+    b.inlet_series_len = 15
+    a.outlet_weight_len = 12
+    rc = L.fmx_adjoint_step(C.byref(d), C.byref(b), 0, 12, 0, 1, C.byref(a), None)  # reads outlet_weight[12]
+    assert rc == abi.ERR_ARG and b"outlet_weight has 12 entries, 13 needed" in L.fmx_last_error()
 
 
 def test_adjoint_state_bytes_and_argument_checks():

@@ -279,6 +279,27 @@ def test_autograd_function(pbb_problem):
     assert abs(loss.item() - pbb_problem.loss()) <= 1e-12 * loss.item()
 
 
+def test_autograd_rejects_a_stale_run(pbb_problem):
+    """Two nodes of one graph on the same AdjointRun: the second forward overwrites the first node's
+    checkpoints and parameters, so the first node's backward must raise instead of returning the
+    second run's gradients; the second node's own backward still works."""
+    import torch
+
+    from fluidmaxxer_amd import abi
+
+    cfg = pbb_problem.cfg
+    run = AdjointRun(cfg, 64, 16, "f64")
+    kl = torch.tensor(cfg.iso_b, dtype=torch.float64)
+    qa = torch.tensor(cfg.iso_a, dtype=torch.float64, requires_grad=True)
+    qb = torch.tensor(cfg.iso_a * 1.1, dtype=torch.float64, requires_grad=True)
+    c1 = CoupledBreakthrough.apply(run, qa, kl, None, None)
+    c2 = CoupledBreakthrough.apply(run, qb, kl, None, None)
+    c2.sum().backward()
+    assert qb.grad is not None and torch.isfinite(qb.grad)
+    with pytest.raises(abi.FmxError, match="belongs to forward run"):
+        c1.sum().backward()
+
+
 def c4_config(steps):
     """C4 geometry (128x64x64 overlapping sphere pack, partial bounce-back, Langmuir) with two
     departures that give the loss something smooth to differentiate:
@@ -367,6 +388,7 @@ def test_adjoint_step_matches_sweep(pbb_problem, prec):
     a.lam_q, a.lam_phi = run.lam_q.data_ptr(), run.lam_phi.data_ptr()
     a.workspace, a.workspace_bytes = run.workspace.data_ptr(), run.workspace.numel()
     a.outlet_weight, a.grad_iso, a.grad_field, a.grad_series = ow.data_ptr(), gi.data_ptr(), gf.data_ptr(), gs.data_ptr()
+    a.outlet_weight_len = ow.numel()
     say(f"  adjoint one step at a time, prec={prec}")
     lam = 0
     for j in range(NSTEPS - 1, -1, -1):

@@ -33,8 +33,8 @@ def vti_array(path, name):
     return np.array(m.group(1).split(), np.float64)
 
 
-@pytest.fixture(scope="module")
-def driver_run(tmp_path_factory):
+@pytest.fixture(scope="module", params=["dirichlet", "pulse"])
+def driver_run(tmp_path_factory, request):
     if not os.path.exists(APP):
         pytest.skip("lbm_coupled not built (python -m fluidmaxxer_amd.build)")
     d = tmp_path_factory.mktemp("driver")
@@ -45,12 +45,14 @@ def driver_run(tmp_path_factory):
     out = str(d / "results")
     cmd = [APP, "--coupling", "--adsorb", "--vti", "--geometry", geo, "--out", out, "--steps", str(STEPS),
            "--every", str(EVERY), "--csv-every", str(CSV_EVERY)]
+    if request.param == "pulse":  # main.cpp:254-271: ADDirichletTanhPulse(0, 1, 0.1 iTmax, 0.9 iTmax, 0.05 iTmax)
+        cmd.append("--pulse")
     say("lbm_coupled", " ".join(cmd[1:]))
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     line = [l for l in r.stdout.splitlines() if l.startswith("params: ")][-1]
     params = json.loads(line[len("params: "):])
-    return dict(dir=out, params=params, mask=pack.sphere, dims=(nx, ny, nz), stdout=r.stdout)
+    return dict(dir=out, params=params, mask=pack.sphere, dims=(nx, ny, nz), stdout=r.stdout, inlet=request.param)
 
 
 def test_driver_outputs(driver_run):
@@ -74,11 +76,17 @@ def test_driver_matches_oracle(driver_run):
     nx, ny, nz = driver_run["dims"]
     inlet, outlet = x_plane(nx, ny, nz, 0), x_plane(nx, ny, nz, nx - 1)
     porosity = np.where(mask == 1, prm["porosity_solid"], 1.0)
+    if driver_run["inlet"] == "pulse":
+        t_on, t_off, k = prm["pulse"]
+        assert 0 < t_on < t_off and k > 0
+        ad_inlet = R.BC(R.AD_TANH, inlet, (0.0, 1.0, k), i0=t_on, i1=t_off)
+    else:
+        ad_inlet = R.BC(R.AD_DIRICHLET, inlet, (1.0,))
     tau_field = np.where(mask == 1, prm["tau_solid"], prm["tau_fluid"])
     s = R.Setup(nx, ny, nz, periodic=(False, True, True), ns_op=R.POROUS, rho0=prm["rho0"], tau=prm["tau"],
                 obstacle=None, porosity=porosity, phi0=0.0, tau_ad=prm["tau_fluid"], minC=1e-8, tau_field=tau_field,
                 ns_bc=[R.BC(R.NS_NEQ, inlet, (prm["u_in"], 0.0, 0.0)), R.BC(R.NS_PRESSURE, outlet, (prm["rho0"],))],
-                ad_bc=[R.BC(R.AD_DIRICHLET, inlet, (1.0,)), R.BC(R.AD_NEUMANN, outlet)], coupling=True, adsorb=True,
+                ad_bc=[ad_inlet, R.BC(R.AD_NEUMANN, outlet)], coupling=True, adsorb=True,
                 isotherm=R.LINEAR, region=mask)
     s.KHenry, s.kA = prm["KHenry"], prm["k_ldf"]
     orc = R.System(s, 8)

@@ -280,3 +280,32 @@ def test_ad_dirichlet_series_and_outlet_stats(prec, oprec, tol):
         o = host(out)
         assert abs(o[0] - vals.mean()) <= 1e-14 * abs(vals.mean()), (o[0], vals.mean())
         assert o[1] == vals.min() and o[2] == vals.max()
+
+
+def test_sync_check_mode():
+    """fmx_set_sync_check(1) restores the reference's synchronise-and-check after every launch
+    (cudaCheckThrow, CudaHelpers.cu:16-20): an operator and a short fused run give the same results
+    as without it, and the mode switches back off."""
+    from fluidmaxxer_amd.coupled import CoupledSim
+    from tests.test_gpu_coupled import small_config
+
+    say("sync-check mode: fused run and ns_collide with FMX sync check on and off")
+    L = abi.lib()
+    cfg = small_config("bgk")
+    results = []
+    dims = (13, 9, 7)
+    n = 13 * 9 * 7
+    f = _rand_f(n, 5)
+    for on in (1, 0):
+        assert L.fmx_set_sync_check(on) == abi.OK
+        sim = CoupledSim(cfg, prec="f32")
+        c = host(sim.run(30))
+        fin = dev(aos_to_soa(f, 19).ravel(), torch().float32)
+        fout = torch().empty_like(fin)
+        g = abi.grid(*dims, (False, True, True))
+        abi.call("fmx_ns_collide_stream", abi.F32, abi.NS_PBB, fin.data_ptr(), fout.data_ptr(), None, None, C.byref(g),
+                 0.62, 1.0, stream())
+        sync()
+        results.append((c, host(sim.phi), host(fout)))
+    for a, b in zip(*results):
+        assert np.array_equal(a, b)

@@ -30,8 +30,11 @@ def run_units(dt, tmp_path):
         pytest.skip("tests/cpp programs not built (python -m fluidmaxxer_amd.build)")
     paths = [str(tmp_path / n) for n in ("a.vti", "b.vti", "out.pvd")]
     r = subprocess.run([exe, *paths], capture_output=True, text=True, check=True)
-    line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
-    return json.loads(line), paths
+    out = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("{"):
+            out.update(json.loads(line))
+    return out, paths
 
 
 def closed_forms():
@@ -114,3 +117,39 @@ def test_vti_reader_binary_forms(tmp_path):
         dims, m = geometry.read_vti(str(path), thr)
         assert dims == (2, 2, 2)
         assert np.array_equal(m, expect), path
+
+
+def test_fp16_decode_all_patterns(tmp_path):
+    """The FP16 decode of the 16-bit population storage (fmx_half.hpp, compiled for the host) against
+    IEEE binary16 for all 65,536 bit patterns: bit-exact for every finite half (subnormals, +-0, the
+    largest 65504), +-Inf decoded as +-Inf and every NaN as a NaN (not as a finite 2^16 (1 + m))."""
+    exe = os.path.join(BUILD, "half_units")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp programs not built (python -m fluidmaxxer_amd.build)")
+    out = tmp_path / "half.f32"
+    subprocess.run([exe, str(out)], check=True)
+    got = np.fromfile(out, dtype=np.float32)
+    ref = np.arange(65536, dtype=np.uint32).astype(np.uint16).view(np.float16).astype(np.float32)
+    finite = np.isfinite(ref)
+    assert finite.sum() == 63488
+    assert np.array_equal(got[finite].view(np.uint32), ref[finite].view(np.uint32))
+    assert np.array_equal(np.isnan(got), np.isnan(ref)) and np.array_equal(np.isinf(got), np.isinf(ref))
+    assert got[0x7c00] == np.inf and got[0xfc00] == -np.inf and got[0x7bff] == 65504.0
+    assert np.array_equal(np.signbit(got[~np.isnan(ref)]), np.signbit(ref[~np.isnan(ref)]))
+
+
+@pytest.mark.parametrize("dt,prec", [("double", 8), ("float", 4)])
+def test_tanh_pulse_value_matches_oracle(dt, prec, tmp_path):
+    """ADDirichletTanhPulse::valueAt (the host class the operator path and CoupledStep's fused series
+    use) against the oracle's restatement of ADDirichletTanhPulse.cu:72-83 -- the pulse in double, cast
+    to DATA_TYPE, times amp, plus base -- at steps across both edges, to the bit; and the pulse's
+    closed-form landmarks: base + amp/2 at t_on and t_off, base + amp on the plateau."""
+    got, _ = run_units(dt, tmp_path)
+    base, amp, t_on, t_off, k = 0.1, 0.9, 300, 900, 40.0
+    for step, v in got["pulse"]:
+        assert v == R.tanh_pulse(prec, step, base, amp, t_on, t_off, k), step
+    vals = dict((s, v) for s, v in got["pulse"])
+    tol = 1e-15 if prec == 8 else 1e-7
+    assert abs(vals[300] - (base + amp * 0.5 * (0.0 - math.tanh(-600 / k)))) < tol
+    assert abs(vals[900] - (base + amp * 0.5 * (math.tanh(600 / k) - 0.0))) < tol
+    assert abs(vals[600] - (base + amp)) < 1e-6 and abs(vals[5000] - base) < 1e-6
