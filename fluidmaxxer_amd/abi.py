@@ -21,6 +21,7 @@ NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
+MASS_COLS = 4
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
@@ -58,6 +59,8 @@ class CoupledBufs(C.Structure):
         ("links", C.c_void_p), ("field", C.c_void_p), ("tau_field", C.c_void_p),
         ("inlet_raw", C.c_void_p), ("inlet_series", C.c_void_p), ("outlet_acc", C.c_void_p),
         ("outlet_ring", C.c_void_p), ("inlet_series_len", C.c_int64), ("outlet_acc_rows", C.c_int64),
+        ("mass_log", C.c_void_p), ("mass_workspace", C.c_void_p), ("mass_log_rows", C.c_int64),
+        ("mass_workspace_bytes", C.c_int64), ("mass_every", C.c_int64),
     ]
 
 
@@ -132,7 +135,8 @@ _SIGS = {
     "fmx_adjoint_step": ([_DP, _BP, _i, _i64, _i, _i, C.POINTER(AdjointBufs), _vp], _i),
     "fmx_adjoint_backward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _vp, _i, _vp, _i64, C.POINTER(AdjointBufs), _vp],
                              _i),
-    "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
+    "fmx_reduce_mass_workspace_bytes": ([_i64], _i64),
+    "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp, _i64, _vp], _i),
     "fmx_reduce_outlet_mean": ([_i, _vp, _vp, _i64, _vp, _vp], _i),
     "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,

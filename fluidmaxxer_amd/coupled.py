@@ -130,6 +130,8 @@ class CoupledSim:
         self.step_index = 0
         self._acc = None
         self._ring = None
+        self._mass_ws = None
+        self.mass_log = None   # (rows, 4) float64 device tensor of the last run's periodic sums, or None
         self.stream = torch.cuda.current_stream(dev)
         abi.call("fmx_coupled_build_links", self._desc_p(), self.flags.data_ptr(), self.links.data_ptr(), self._s())
         abi.call("fmx_coupled_init", self._desc_p(), self._bufs_p(), self.cur, float(cfg.phi0), self._s())
@@ -146,7 +148,13 @@ class CoupledSim:
     def _desc_p(self):
         return abi.C.byref(self._desc)
 
-    def _bufs(self, acc=None) -> abi.CoupledBufs:
+    def _mass_workspace(self):
+        if self._mass_ws is None:
+            nb = int(abi.lib().fmx_reduce_mass_workspace_bytes(self.cfg.n))
+            self._mass_ws = _torch().empty(nb, dtype=_torch().uint8, device=self.device)
+        return self._mass_ws
+
+    def _bufs(self, acc=None, mass=None, mass_every=0) -> abi.CoupledBufs:
         b = abi.CoupledBufs()
         b.f[0], b.f[1] = self.f[0].data_ptr(), self.f[1].data_ptr()
         b.g[0], b.g[1] = self.g[0].data_ptr(), self.g[1].data_ptr()
@@ -164,10 +172,14 @@ class CoupledSim:
                 self._ring = _torch().empty(abi.OUTLET_RING_ROWS * self.cfg.ny * self.cfg.nz,
                                             dtype=_torch().float64, device=self.device)
             b.outlet_ring = self._ring.data_ptr()
+        if mass is not None:
+            ws = self._mass_workspace()
+            b.mass_log, b.mass_log_rows, b.mass_every = mass.data_ptr(), mass.shape[0], int(mass_every)
+            b.mass_workspace, b.mass_workspace_bytes = ws.data_ptr(), ws.numel()
         return b
 
-    def _bufs_p(self, acc=None):
-        self._b = self._bufs(acc)
+    def _bufs_p(self, acc=None, mass=None, mass_every=0):
+        self._b = self._bufs(acc, mass, mass_every)
         return abi.C.byref(self._b)
 
     # ------------------------------------------------------------- stepping
@@ -181,15 +193,21 @@ class CoupledSim:
         abi.call("fmx_fill_zero", acc.data_ptr(), need * 8, self._s())
         return acc
 
-    def run(self, nsteps: int, log_outlet: bool = True, acc=None):
-        """Advance `nsteps` steps; returns the per-step outlet mean (float64 device tensor) or None."""
+    def run(self, nsteps: int, log_outlet: bool = True, acc=None, mass_every: int = 0):
+        """Advance `nsteps` steps; returns the per-step outlet mean (float64 device tensor) or None.
+        With mass_every > 0, self.mass_log holds one row (sum phi, sum q, sum of stored f, non-finite
+        count) after every mass_every-th step of this run."""
         torch = _torch()
         if self.inlet_series is not None and self.step_index + nsteps >= self.inlet_series.numel():
             raise abi.FmxError("inlet_series is shorter than the requested steps")
         if log_outlet and acc is None:
             acc = self.outlet_acc(nsteps)
-        abi.call("fmx_coupled_run", self._desc_p(), self._bufs_p(acc if log_outlet else None), self.step_index + 1,
-                 nsteps, self.cur, self._s())
+        mass = None
+        if mass_every > 0:
+            mass = torch.zeros((max(nsteps // mass_every, 1), abi.MASS_COLS), dtype=torch.float64, device=self.device)
+        self.mass_log = mass
+        abi.call("fmx_coupled_run", self._desc_p(), self._bufs_p(acc if log_outlet else None, mass, mass_every),
+                 self.step_index + 1, nsteps, self.cur, self._s())
         self.cur = (self.cur + nsteps) & 1
         self.step_index += nsteps
         if not log_outlet:
@@ -250,11 +268,12 @@ class CoupledSim:
         self.stream.synchronize()
 
     def mass(self):
-        """FP64 (sum phi, sum q, sum rho of the stored f) over all cells."""
+        """FP64 (sum phi, sum q, sum rho of the stored f, non-finite count) over all cells."""
         torch = _torch()
-        out = torch.zeros(3, dtype=torch.float64, device=self.device)
+        out = torch.zeros(abi.MASS_COLS, dtype=torch.float64, device=self.device)
+        ws = self._mass_workspace()
         abi.call("fmx_reduce_mass", self.prec, self.phi.data_ptr(), self.q.data_ptr(), self.f[self.cur].data_ptr(),
-                 self.cfg.n, float(self.cfg.f_ref), out.data_ptr(), self._s())
+                 self.cfg.n, float(self.cfg.f_ref), out.data_ptr(), ws.data_ptr(), ws.numel(), self._s())
         return out
 
     def synchronize(self):

@@ -35,6 +35,7 @@
 // When x = nx-2 and nx-1 fall in different waves ((nx - 1) % 64 == 0) the x = 1 and x = nx-2
 // threads write the neighbouring boundary cell's populations instead.
 #include "coupled_kernel.hpp"
+#include "diagnostics.hpp"
 
 namespace fmx {
 
@@ -301,6 +302,14 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
             if (int rc = after_launch("k_outlet_ring_rows")) return rc;
             rows = 0;
         }
+        const i64 done = acc_row0 + k + 1;  // steps of the caller's run completed
+        if (b.mass_log && b.mass_every > 0 && done % b.mass_every == 0) {
+            const int nxt = (cur + k + 1) & 1;  // the buffers this step wrote
+            if (int rc = launch_mass_any(d.prec, b.phi[nxt], b.q, b.f[nxt], prm.n, d.f_ref,
+                                         static_cast<double *>(b.mass_workspace),
+                                         b.mass_log + (done / b.mass_every - 1) * kMassCols, s))
+                return rc;
+        }
     }
     return FMX_OK;
 }
@@ -395,6 +404,17 @@ int check_run_buffers(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
     if (writes_outlet && b.outlet_acc && b.outlet_acc_rows < nsteps)
         return set_error(FMX_ERR_ARG, "%s: %lld steps write %lld outlet_acc rows, but outlet_acc_rows is %lld", who,
                          (long long)nsteps, (long long)nsteps, (long long)b.outlet_acc_rows);
+    if (writes_outlet && b.mass_log) {
+        if (b.mass_every < 1) return set_error(FMX_ERR_ARG, "%s: mass_log needs mass_every >= 1", who);
+        const i64 n = i64(d.nx) * d.ny * d.nz;
+        const i64 need = i64(mass_blocks(n)) * kMassCols * i64(sizeof(double));
+        if (!b.mass_workspace || b.mass_workspace_bytes < need)
+            return set_error(FMX_ERR_ARG, "%s: mass_log needs a mass_workspace of %lld bytes (%lld given)", who,
+                             (long long)need, (long long)(b.mass_workspace ? b.mass_workspace_bytes : 0));
+        if (b.mass_log_rows < nsteps / b.mass_every)
+            return set_error(FMX_ERR_ARG, "%s: %lld steps log %lld mass rows, but mass_log_rows is %lld", who,
+                             (long long)nsteps, (long long)(nsteps / b.mass_every), (long long)b.mass_log_rows);
+    }
     return FMX_OK;
 }
 

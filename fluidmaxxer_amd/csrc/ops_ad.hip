@@ -206,60 +206,6 @@ __global__ void __launch_bounds__(256) k_plane_copy(const T *__restrict__ src, T
     for (int k = 0; k < K; ++k) dst[i64(k) * n + c] = v[k];
 }
 
-// ---------------------------------------------------------- diagnostics
-// Deterministic two-pass FP64 sums.  Pass 1: each block reduces a fixed
-// contiguous chunk to one partial per quantity; pass 2: one block adds the
-// partials in block order.
-template <int P>
-__global__ void __launch_bounds__(256)
-    k_mass_partial(const typename Storage<P>::C *__restrict__ phi, const typename Storage<P>::C *__restrict__ q,
-                   const typename Storage<P>::S *__restrict__ f, i64 n, typename Storage<P>::C fref, i64 chunk,
-                   double *__restrict__ partial) {
-    using C = typename Storage<P>::C;
-    __shared__ double red[3][256];
-    const i64 begin = i64(blockIdx.x) * chunk;
-    const i64 end = begin + chunk < n ? begin + chunk : n;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (i64 c = begin + threadIdx.x; c < end; c += blockDim.x) {
-        if (phi) s0 += double(phi[c]);
-        if (q) s1 += double(q[c]);
-        if (f) {
-            C r = C(0);
-#pragma unroll
-            for (int k = 0; k < 19; ++k) r += ld_pop<P>(f, k * n + c, w19<C>(k), fref);
-            s2 += double(r);
-        }
-    }
-    red[0][threadIdx.x] = s0;
-    red[1][threadIdx.x] = s1;
-    red[2][threadIdx.x] = s2;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (int(threadIdx.x) < h)
-            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) partial[blockIdx.x * 3 + k] = red[k][0];
-}
-
-__global__ void __launch_bounds__(256) k_mass_final(const double *__restrict__ partial, int nblocks,
-                                                    double *__restrict__ out) {
-    __shared__ double red[3][256];
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-        for (int k = 0; k < 3; ++k) s[k] += partial[b * 3 + k];
-    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (int(threadIdx.x) < h)
-            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) out[k] = red[k][0];
-}
-
 template <class T> static int ad_collide(const void *g, void *gn, void *phi, const void *tau, const void *ux,
                                          const void *uy, const void *uz, const void *S, const fmx_grid &gr,
                                          double minC, hipStream_t s) {
@@ -279,26 +225,6 @@ static int ldf(i64 n, const std::uint8_t *region, const void *phi, void *S, void
     return after_launch("k_ldf");
 }
 
-template <int P>
-static int reduce_mass(const void *phi, const void *q, const void *f, i64 n, double fref, double *out, hipStream_t s) {
-    using C = typename Storage<P>::C;
-    using S = typename Storage<P>::S;
-    const i64 chunk = 1 << 16;
-    const int nblocks = int((n + chunk - 1) / chunk);
-    double *partial = nullptr;
-    int rc = check_hip(hipMallocAsync(reinterpret_cast<void **>(&partial), sizeof(double) * 3 * nblocks, s),
-                       "hipMallocAsync");
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_mass_partial<P>, dim3(nblocks), dim3(256), 0, s, static_cast<const C *>(phi),
-                       static_cast<const C *>(q), static_cast<const S *>(f), n, C(fref), chunk, partial);
-    rc = after_launch("k_mass_partial");
-    if (!rc) {
-        hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
-        rc = after_launch("k_mass_final");
-    }
-    int rc2 = check_hip(hipFreeAsync(partial, s), "hipFreeAsync");
-    return rc ? rc : rc2;
-}
 
 }  // namespace fmx
 
@@ -515,19 +441,5 @@ int fmx_stream_probe(int mode, void *dst, const void *src, size_t bytes, void *s
     FMX_TRY_END
 }
 
-int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
-                    void *stream) {
-    FMX_TRY_BEGIN
-    if (n <= 0 || !out) return set_error(FMX_ERR_ARG, "fmx_reduce_mass: bad arguments");
-    hipStream_t s = as_stream(stream);
-    switch (prec) {
-        case FMX_F32: return reduce_mass<FMX_F32>(phi, q, f, n, f_ref, out, s);
-        case FMX_F64: return reduce_mass<FMX_F64>(phi, q, f, n, f_ref, out, s);
-        case FMX_F16: return reduce_mass<FMX_F16>(phi, q, f, n, f_ref, out, s);
-        case FMX_BF16: return reduce_mass<FMX_BF16>(phi, q, f, n, f_ref, out, s);
-        default: return set_error(FMX_ERR_ARG, "fmx_reduce_mass: unknown precision %d", prec);
-    }
-    FMX_TRY_END
-}
 
 }  // extern "C"

@@ -27,7 +27,8 @@ extern "C" {
 #endif
 
 /* 2: buffer lengths in fmx_coupled_bufs (inlet_series_len, outlet_acc_rows) and fmx_adjoint_bufs
- * (outlet_weight_len), checked before any launch */
+ * (outlet_weight_len), checked before any launch; fmx_reduce_mass with a caller workspace and a
+ * non-finite count, and its periodic log inside fmx_coupled_run (mass_log) */
 #define FMX_ABI_VERSION 2
 
 /* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
@@ -202,6 +203,13 @@ typedef struct fmx_coupled_bufs {
     int64_t inlet_series_len; /* entries of inlet_series: a run over steps step0 .. step0+nsteps-1 reads
                                  entries step0 .. step0+nsteps-1, so it needs step0 + nsteps of them */
     int64_t outlet_acc_rows;  /* rows of outlet_acc: a run of nsteps steps writes rows 0 .. nsteps-1 */
+    double *mass_log;         /* nullable: every mass_every-th step of a run appends one row of FMX_MASS_COLS
+                                 doubles, fmx_reduce_mass of the state after that step (sum phi, sum q, sum
+                                 of the stored f, non-finite count); row r after step step0 + (r+1)*mass_every - 1 */
+    void *mass_workspace;     /* with mass_log: fmx_reduce_mass_workspace_bytes(nx*ny*nz) bytes of scratch */
+    int64_t mass_log_rows;    /* rows of mass_log (>= nsteps / mass_every) */
+    int64_t mass_workspace_bytes;
+    int64_t mass_every;       /* steps between mass_log rows (>= 1 with mass_log) */
 } fmx_coupled_bufs;
 #define FMX_OUTLET_RING_ROWS 64
 
@@ -322,10 +330,16 @@ int fmx_adjoint_step(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs,
                      int first, const fmx_adjoint_bufs *adj, void *stream);
 
 /* ---------------------------------------------------------- diagnostics */
-/* FP64 sums over all cells: out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all f);
- * f is SoA in `prec` storage (half storage needs f_ref); any input may be NULL (sum = 0) */
+/* No counterpart in the reference (its only reduction is OutletCSV::log's host loop, OutletCSV.hpp:29-58).
+ * FP64 sums over all cells as wave64 reductions in a fixed order (a call repeats to the bit):
+ * out[0] = sum phi, out[1] = sum q, out[2] = sum rho (= sum of all stored f), out[3] = number of
+ * non-finite values seen (phi, q, or a cell whose populations sum to Inf / NaN).  f is SoA in `prec`
+ * storage (half storage needs f_ref); phi / q in its compute precision; any input may be NULL (sum 0).
+ * `workspace`: fmx_reduce_mass_workspace_bytes(n) bytes of device scratch (the per-block partials). */
+#define FMX_MASS_COLS 4
+int64_t fmx_reduce_mass_workspace_bytes(int64_t n);
 int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
-                    void *stream);
+                    void *workspace, int64_t workspace_bytes, void *stream);
 /* OutletCSV::log (OutletCSV.hpp:29-58) on the device: out[0] = mean, out[1] = min, out[2] = max of phi
  * (compute precision of `prec`) over the listed cells, in FP64 and a fixed summation order */
 int fmx_reduce_outlet_mean(int prec, const void *phi, const int64_t *cells, int64_t count, double *out, void *stream);

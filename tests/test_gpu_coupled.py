@@ -277,6 +277,60 @@ def test_mass_reduction_matches_host_sums():
     assert np.isclose(m[0], host(sim.phi).sum(), rtol=1e-13)
     assert np.isclose(m[1], host(sim.q).sum(), rtol=1e-13)
     assert np.isclose(m[2], f.sum(), rtol=1e-13)
+    assert m[3] == 0
+
+
+@pytest.mark.parametrize("variant,rho_too", [("no_inlet_outlet", True), ("langmuir_series", False)])
+def test_mass_log_matches_oracle_sums(variant, rho_too):
+    """The periodic mass log of fmx_coupled_run (a row after every 5th step) against FP64 sums of the
+    oracle's state after the same steps: sum phi, sum q and -- where no inlet plane holds the next
+    step's inlet update -- sum of the NS populations.  The FP64 fused step is bit-identical to the FP64
+    oracle, so only the summation order differs (relative 1e-13).  The last row equals fmx_reduce_mass
+    of the final state to the bit; no non-finite values."""
+    cfg = small_config(variant)
+    every, steps = 5, 30
+    say(f"mass log {variant}: {steps} steps, a row every {every}")
+    sim = CoupledSim(cfg, prec="f64")
+    sim.run(steps, mass_every=every)
+    log = host(sim.mass_log)
+    assert log.shape == (steps // every, 4)
+    assert np.array_equal(log[-1], host(sim.mass()))
+    orc = R.System(oracle_setup(cfg, R), 8)
+    for r in range(steps // every):
+        orc.run(every)
+        ref = [orc.get(R.PHI).sum(), orc.get(R.Q).sum(), orc.get(R.F).sum()]
+        assert abs(log[r, 0] - ref[0]) <= 1e-13 * abs(ref[0]), (r, log[r], ref)
+        assert abs(log[r, 1] - ref[1]) <= 1e-13 * max(abs(ref[1]), 1e-300), (r, log[r], ref)
+        if rho_too:
+            assert abs(log[r, 2] - ref[2]) <= 1e-13 * abs(ref[2]), (r, log[r], ref)
+        assert log[r, 3] == 0
+    assert log[-1, 1] > 0  # sorption happened
+
+
+def test_mass_counts_non_finite_values():
+    """A population beyond the FP16 range is stored as Inf and a NaN concentration stays NaN: the
+    diagnostic counts one non-finite cell each (FP16 and FP32 storage), and the sums of two calls
+    agree to the bit."""
+    import torch
+
+    cfg = small_config("eq_inlet")
+    n = cfg.n
+    say("mass diagnostic: non-finite count, FP16 overflow and a NaN concentration")
+    sim = CoupledSim(cfg, prec="f16")
+    f = np.outer(W19, np.full(n, cfg.rho0))
+    f[3, 77] = 1e6 * W19[3]          # f/w - f_ref ~ 1e6 > 65504: encodes to +Inf
+    sim.load(f=f)
+    m = host(sim.mass())
+    assert m[3] == 1 and np.isinf(m[2]) and np.isfinite(m[0])
+    sim32 = CoupledSim(cfg, prec="f32")
+    sim32.run(3)
+    a = host(sim32.mass())
+    assert np.array_equal(a, host(sim32.mass())) and a[3] == 0
+    phi = host(sim32.phi).copy()
+    phi[5] = np.nan
+    sim32.phi.copy_(torch.as_tensor(phi, device=sim32.device))
+    b = host(sim32.mass())
+    assert b[3] == 1 and np.isnan(b[0]) and b[1] == a[1] and b[2] == a[2]
 
 
 @pytest.mark.parametrize("periodic", [(True, True), (False, True), (False, False)])
