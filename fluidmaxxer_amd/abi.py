@@ -137,6 +137,7 @@ _SIGS = {
                              _i),
     "fmx_reduce_mass_workspace_bytes": ([_i64], _i64),
     "fmx_reduce_mass": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp, _i64, _vp], _i),
+    "fmx_reduce_mass_static": ([_i, _vp, _vp, _vp, _i64, _d, _vp, _vp], _i),
     "fmx_reduce_outlet_mean": ([_i, _vp, _vp, _i64, _vp, _vp], _i),
     "fmx_stream_probe": ([_i, _vp, _vp, _sz, _vp], _i),
     "fmx_geom_sphere_pack": ([C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _d, _d, _d, C.c_int32, C.c_int32,

@@ -8,6 +8,7 @@
 // FP64 too), then a wave64 butterfly of shuffles, then the block's four wave sums in wave order -- into
 // four partials; one block then adds the partials in block order the same way.
 #include <cmath>
+#include <mutex>
 
 #include "coupled_common.hpp"
 #include "diagnostics.hpp"
@@ -118,6 +119,38 @@ int launch_mass_any(int prec, const void *phi, const void *q, const void *f, i64
     }
 }
 
+// ---------------------------------------------------------------- no-allocation variant
+// The partials of fmx_reduce_mass_static: the largest partial-sum grid (kMassMaxBlocks blocks), 64 KiB
+// per GPU, held until the process ends.  In host code `scratch` names a host-side stand-in, not a device
+// address: the kernels get the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[kMassMaxBlocks * kMassCols];
+
+constexpr int kMaxDevices = 64;
+// cells the static scratch serves: its kMassMaxBlocks blocks of at most 2^24 cells each
+constexpr i64 kMassStaticMaxCells = i64(kMassMaxBlocks) << 24;
+
+// Device address of `scratch` on the current device: looked up once per device (with that device
+// current, on its first call) and kept.
+static int scratch_address(double **out) {
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    int dev = 0;
+    if (int rc = check_hip(hipGetDevice(&dev), "fmx_reduce_mass_static: hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(FMX_ERR_UNSUPPORTED, "fmx_reduce_mass_static: device %d beyond the %d supported", dev,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                               "fmx_reduce_mass_static: hipGetSymbolAddress(scratch)"))
+            return rc;
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return FMX_OK;
+}
+
 }  // namespace fmx
 
 using namespace fmx;
@@ -139,6 +172,22 @@ int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int
                                       "(fmx_reduce_mass_workspace_bytes)",
                          (long long)(workspace ? workspace_bytes : 0), (long long)need);
     return launch_mass_any(prec, phi, q, f, n, f_ref, static_cast<double *>(workspace), out, as_stream(stream));
+    FMX_TRY_END
+}
+
+int fmx_reduce_mass_static(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref,
+                           double *out, void *stream) {
+    FMX_TRY_BEGIN
+    if (n <= 0 || !out) return set_error(FMX_ERR_ARG, "fmx_reduce_mass_static: bad arguments");
+    if (n > kMassStaticMaxCells)
+        return set_error(FMX_ERR_ARG, "fmx_reduce_mass_static: %lld cells exceed the %lld its static scratch serves "
+                                      "(use fmx_reduce_mass with a workspace)",
+                         (long long)n, (long long)kMassStaticMaxCells);
+    if (mass_blocks(n) * kMassCols > kMassMaxBlocks * kMassCols)
+        return set_error(FMX_ERR_ARG, "fmx_reduce_mass_static: partial grid larger than the static scratch");
+    double *partial = nullptr;
+    if (int rc = scratch_address(&partial)) return rc;
+    return launch_mass_any(prec, phi, q, f, n, f_ref, partial, out, as_stream(stream));
     FMX_TRY_END
 }
 

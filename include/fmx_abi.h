@@ -340,6 +340,13 @@ int fmx_adjoint_step(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs,
 int64_t fmx_reduce_mass_workspace_bytes(int64_t n);
 int fmx_reduce_mass(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref, double *out,
                     void *workspace, int64_t workspace_bytes, void *stream);
+/* No counterpart in the reference.  fmx_reduce_mass for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc): the same kernels and the same bits,
+ * with the partials in a 64 KiB array the library holds per GPU until the process ends instead of a
+ * workspace argument.  Allocates nothing; at most 2^35 cells (FMX_ERR_ARG beyond).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (different streams, threads) race. */
+int fmx_reduce_mass_static(int prec, const void *phi, const void *q, const void *f, int64_t n, double f_ref,
+                           double *out, void *stream);
 /* OutletCSV::log (OutletCSV.hpp:29-58) on the device: out[0] = mean, out[1] = min, out[2] = max of phi
  * (compute precision of `prec`) over the listed cells, in FP64 and a fixed summation order */
 int fmx_reduce_outlet_mean(int prec, const void *phi, const int64_t *cells, int64_t count, double *out, void *stream);
