@@ -253,6 +253,66 @@ def adjoint_c5_accuracy(steps=None, k=500):
     }
 
 
+def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", "f16", "bf16")):
+    """The workload run for `steps` steps (C3: 512^3, 20000) in each storage precision, against the FP64
+    run: per-field errors after the last step (rho, u of the populations as the reference holds them, C,
+    q), the outlet breakthrough curve C_out(t) of every step, the breakthrough surface C(x, t) -- the
+    plane-averaged concentration of every x-plane every `every` steps, which carries the front before it
+    reaches the outlet -- and the FP64 sums of C and q (the diagnostic log) at the same times.  Errors:
+    max |a - b| / max |b| (and the relative l2 norm for the fields)."""
+    import torch
+
+    from fluidmaxxer_amd import workloads
+    from fluidmaxxer_amd.coupled import CoupledSim
+
+    nsteps = steps or workloads.SPECS[workload]["steps"]
+    every = max(1, min(every, nsteps))
+    nsteps -= nsteps % every
+    cfg = workloads.build(workload, steps=nsteps + 1)
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    ref = None
+    out = {"workload": f"{workload}: {nx}x{ny}x{nz} {cfg.ns_op} {cfg.isotherm}, porosity {cfg.meta['porosity']:.4f}, "
+                       f"{nsteps} steps from the equilibrium start, C(x, t) every {every} steps"}
+
+    def linf(a, b):
+        return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+    def l2(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b).clamp_min(1e-300))
+
+    for prec in precs:
+        t0 = time.perf_counter()
+        sim = CoupledSim(cfg, prec=prec)
+        curve, surface, mass = [], [], []
+        for blk in range(nsteps // every):
+            curve.append(sim.run(every, mass_every=every))
+            surface.append(sim.phi.view(nz, ny, nx).double().mean(dim=(0, 1)))
+            mass.append(sim.mass_log[0].clone())
+            print(f"[precision] {prec} step {(blk + 1) * every}/{nsteps} C_out {float(curve[-1][-1]):.3e} "
+                  f"front C(x={nx // 4}) {float(surface[-1][nx // 4]):.3e} ({time.perf_counter() - t0:.0f} s)",
+                  flush=True)
+        curve, surface, mass = torch.cat(curve), torch.stack(surface), torch.stack(mass)
+        rho, ux, uy, uz = sim.moments()
+        fields = {"rho": rho.double(), "ux": ux.double(), "uy": uy.double(), "uz": uz.double(),
+                  "C": sim.phi.double().clone(), "q": sim.q.double().clone()}
+        del rho, ux, uy, uz, sim
+        torch.cuda.empty_cache()
+        res = {"seconds": round(time.perf_counter() - t0, 1), "c_out_last": float(curve[-1]),
+               "front_c_quarter": float(surface[-1][nx // 4]), "nonfinite": float(mass[:, 3].sum())}
+        if ref is None:
+            ref = dict(fields=fields, curve=curve, surface=surface, mass=mass)
+        else:
+            res["fields_linf"] = {k: round(linf(v, ref["fields"][k]), 9) for k, v in fields.items()}
+            res["fields_l2"] = {k: round(l2(v, ref["fields"][k]), 9) for k, v in fields.items()}
+            res["c_out_curve"] = linf(curve, ref["curve"])
+            res["breakthrough_surface"] = linf(surface, ref["surface"])
+            res["sum_C_plus_q"] = linf(mass[:, 0] + mass[:, 1], ref["mass"][:, 0] + ref["mass"][:, 1])
+            del fields
+        out[prec] = res
+        print(f"[precision] {prec}: {json.dumps(res)}", flush=True)
+    return out
+
+
 def time_steps(sim, steps, warmup):
     """Warm-up steps, then exactly `steps` launches between two HIP events on the launch stream (with the
     per-step outlet reduction); returns ms per step."""
@@ -328,6 +388,9 @@ def main():
                     help="steps of the adjoint timing (default: the workload's own)")
     ap.add_argument("--adjoint-c5-accuracy", action="store_true",
                     help="only C5 at full size: FP16-checkpoint gradients against the FP64 adjoint")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="only the storage-precision study: the workload (default C3, 512^3) for its own steps "
+                         "(20000) in FP64, FP32, FP16 and BF16, errors against FP64")
     ap.add_argument("--adjoint-c5", action="store_true",
                     help="only the adjoint/forward timing on C5 as specified (256x128x128, 20000 steps, FP16 "
                          "checkpoints every 500 steps, FP32 recompute; one-level and two-level segment stores)")
@@ -346,6 +409,10 @@ def main():
 
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
+        return
+
+    if args.precision_study:
+        print(json.dumps({"precision_study": precision_study(args.workload, args.steps)}), flush=True)
         return
 
     if args.adjoint_c5_accuracy:

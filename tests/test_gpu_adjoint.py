@@ -66,11 +66,11 @@ def adj_config(op="pbb", series=True, closed=False, seed=21, dims=(16, 10, 10), 
 class Problem:
     """Loss, adjoint gradients and finite differences for one configuration."""
 
-    def __init__(self, cfg, prec="f64", nsteps=NSTEPS, k=K, target=None, ckpt=None):
+    def __init__(self, cfg, prec="f64", nsteps=NSTEPS, k=K, target=None, ckpt=None, seg_slots=None):
         say(f"adjoint problem prec={prec} ckpt={ckpt or prec} op={cfg.ns_op} dims={(cfg.nx, cfg.ny, cfg.nz)} "
-            f"steps={nsteps} k={k}")
+            f"steps={nsteps} k={k} seg_slots={seg_slots}")
         self.cfg = cfg
-        self.run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt)
+        self.run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt, seg_slots=seg_slots)
         self.base = dict(iso_a=cfg.iso_a, iso_b=cfg.iso_b,
                          field=None if cfg.field is None else np.asarray(cfg.field, np.float64).copy(),
                          series=None if cfg.inlet_series is None else np.asarray(cfg.inlet_series, np.float64).copy())
@@ -587,3 +587,60 @@ def test_c5_half_checkpoints_against_fp64():
         f"|dL/dknots| {np.linalg.norm(k64):.3e}")
     assert errs["c_out"] < 1e-3 and errs["knots"] < 3e-3 and errs["field"] < 3e-3, errs
     assert errs["iso_a"] < 3e-3 and errs["iso_b"] < 3e-3, errs
+
+
+def test_c5_full_size_fp16_checkpoints_against_fp64():
+    """C5 at its own size: the 256x128x128 overlapping sphere pack at porosity 0.4 (seed 5), partial
+    bounce-back, Langmuir, the 200-knot inlet series, 20000 steps; the FP32 run with FP16-stored
+    checkpoints every 500 steps and FP32 recompute against the FP64 adjoint (both with two-level
+    segment stores, so the FP64 store is about 80 GB), and against the FP32 run with FP32 checkpoints.
+    As in the reduced-size test the fluid cells' solid fraction and the concentration clamp are 0.
+
+    The concentration front has NOT reached the outlet by step 20000 (C_out ~ 1e-15, asserted): the loss
+    sum (C_out - 0.85 C_out^FP64)^2 weighs only the front's far tail, fifteen orders of magnitude below
+    the inlet value.  There the FP32 forward run itself is about 0.6% off the FP64 curve and its
+    gradients about 4.5% off the FP64 adjoint's, with FP32 and FP16 checkpoints alike, so the FP16
+    checkpoints' own effect is measured against the FP32-checkpoint run (the forward curves are the same
+    bits).  Bounds: about 2x the measured errors."""
+    import torch
+
+    steps, k = 20000, 500
+    cfg = workloads.build("C5", steps=steps)
+    assert (cfg.nx, cfg.ny, cfg.nz) == (256, 128, 128) and len(cfg.meta["knots"]) == 200
+    cfg.field = np.where(cfg.field > 0.5, cfg.field, 0.0)
+    cfg.min_conc = 0.0
+    knots = cfg.meta["knots"]
+    B = interpolation_matrix(len(knots), steps)
+    from fluidmaxxer_amd import abi
+
+    def grads(prec, ckpt, target):
+        seg = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+        p = Problem(cfg, prec=prec, nsteps=steps, k=k, target=target, ckpt=ckpt, seg_slots=seg)
+        if target is None:
+            p.target = 0.85 * p.c0
+        g = p.grads()
+        out = dict(c=p.c0, target=p.target, knots=B.T @ g["series"], field=g["field"], iso_a=g["iso_a"],
+                   iso_b=g["iso_b"])
+        del p, g
+        torch.cuda.empty_cache()
+        return out
+
+    g64 = grads("f64", None, None)
+    say(f"  C5 256x128x128 FP64 c_out at 5000/10000/20000: {[float('%.3e' % g64['c'][s - 1]) for s in (5000, 10000, 20000)]}")
+    assert g64["c"][-1] < 1e-10  # the front is still inside the pack
+    g16 = grads("f32", "f16", g64["target"])
+    g32 = grads("f32", None, g64["target"])
+
+    def errs(a, b):
+        return dict(knots=rel_l2(a["knots"], b["knots"]), field=rel_l2(a["field"], b["field"]),
+                    c_out=rel_l2(a["c"], b["c"]), iso_a=rel(a["iso_a"], b["iso_a"]), iso_b=rel(a["iso_b"], b["iso_b"]))
+
+    e16_64, e32_64, e16_32 = errs(g16, g64), errs(g32, g64), errs(g16, g32)
+    for name, e in (("f32 run + f16 ckpt vs f64", e16_64), ("f32 run + f32 ckpt vs f64", e32_64),
+                    ("f16 ckpt vs f32 ckpt", e16_32)):
+        say(f"  C5 {name}: {({kk: f'{v:.2e}' for kk, v in e.items()})}")
+    assert np.array_equal(g16["c"], g32["c"])  # 16-bit checkpoints do not touch the forward run
+    assert e16_64["c_out"] < 1.2e-2
+    assert max(e16_64[n] for n in ("knots", "field", "iso_a", "iso_b")) < 9e-2, e16_64
+    assert e16_32["knots"] < 1.6e-3 and e16_32["field"] < 9e-3, e16_32
+    assert e16_32["iso_a"] < 8.4e-4 and e16_32["iso_b"] < 8.4e-4, e16_32
