@@ -3,11 +3,13 @@ loss L = sum_s (C_out(s) - C*(s))^2 with respect to the Langmuir constants (q_ma
 the per-cell partial-bounce-back fraction / porosity and the per-step inlet
 concentration, checked against
 
-  * central finite differences of the GPU forward run in FP64 (relative 1e-6; the
-    FP64 fused step is bit-identical to the FP64 oracle),
+  * central finite differences of the GPU forward run in FP64 (observed 1e-10 .. 1e-7 relative,
+    each bound about 3x its observation; the FP64 fused step is bit-identical to the FP64 oracle),
   * central finite differences of the CPU oracle (the restatement of the reference
-    update) in FP64 (relative 1e-6),
-  * the FP64 adjoint, for the FP32 adjoint (relative 1e-3),
+    update) in FP64 (observed 4e-10),
+  * the FP64 adjoint, for the FP32 adjoint (6e-6 on the small pack, 2e-3 at C4 scale over 12000
+    steps: the FP32 forward run's own drift from FP64 -- reverse steps taken in FP32 on the FP64
+    run's states agree with the FP64 adjoint to 1e-7 .. 1e-5),
   * itself under a different checkpoint interval (relative 1e-12),
   * the FP32-checkpoint adjoint, for an FP32 run with 16-bit checkpoints and FP32 recompute,
 and used through the torch.autograd.Function wrapper.
@@ -120,7 +122,7 @@ def test_iso_grads_match_fd_pbb(pbb_problem):
     for name in ("iso_a", "iso_b"):
         fd = pbb_problem.fd_scalar(name, 1e-5 * pbb_problem.base[name])
         say(f"  d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 1.5e-9  # observed 4.3e-10
 
 
 def test_field_grads_match_fd_pbb(pbb_problem):
@@ -131,13 +133,13 @@ def test_field_grads_match_fd_pbb(pbb_problem):
     fd = pbb_problem.fd_dir("field", v, 1e-6)
     ad = float(gf @ v)
     say(f"  <dL/dfield, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
-    assert rel(ad, fd) < 1e-6
+    assert rel(ad, fd) < 3e-8  # observed 9.5e-9
     for c in np.argsort(-np.abs(gf))[:3]:
         e = np.zeros(pbb_problem.cfg.n)
         e[c] = 1.0
         fdc = pbb_problem.fd_dir("field", e, 1e-6)
         say(f"  d L/d field[{c}]: adjoint {gf[c]:.12e} fd {fdc:.12e} rel {rel(gf[c], fdc):.2e}")
-        assert rel(gf[c], fdc) < 1e-5
+        assert rel(gf[c], fdc) < 5e-9  # observed <= 1.6e-9
 
 
 def test_series_grads_match_fd(pbb_problem):
@@ -149,12 +151,13 @@ def test_series_grads_match_fd(pbb_problem):
     fd = pbb_problem.fd_dir("series", v, 1e-6)
     ad = float(gs @ v)
     say(f"  <dL/dseries, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
-    assert rel(ad, fd) < 1e-6
+    assert rel(ad, fd) < 1.5e-9  # observed 4.9e-10
     for s in np.argsort(-np.abs(gs))[:3]:
         e = np.zeros(gs.size)
         e[s] = 1.0
         fds = pbb_problem.fd_dir("series", e, 1e-5)
-        assert rel(gs[s], fds) < 1e-5, (s, gs[s], fds)
+        say(f"  d L/d series[{s}]: adjoint {gs[s]:.12e} fd {fds:.12e} rel {rel(gs[s], fds):.2e}")
+        assert rel(gs[s], fds) < 1.5e-9, (s, gs[s], fds)  # observed <= 4.6e-10
 
 
 @pytest.mark.parametrize("op", ["bgk", "porous"])
@@ -165,13 +168,13 @@ def test_grads_match_fd_other_operators(op):
     for name in ("iso_a", "iso_b"):
         fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
         say(f"  {op} d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 1.5e-9  # observed <= 3.7e-10
     if op == "porous":
         v = np.random.default_rng(5).standard_normal(pr.cfg.n)
         fd = pr.fd_dir("field", v, 1e-6)
         ad = float(g["field"] @ v)
         say(f"  porous <dL/dfield, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
-        assert rel(ad, fd) < 1e-6
+        assert rel(ad, fd) < 1e-9  # observed 2.5e-10
 
 
 def test_grads_match_fd_closed_box():
@@ -183,7 +186,7 @@ def test_grads_match_fd_closed_box():
     for name in ("iso_a", "iso_b"):
         fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
         say(f"  closed d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 5e-10  # observed 1.3e-10
     assert np.max(np.abs(g["field"])) < 1e-12 * abs(g["iso_a"])
 
 
@@ -195,13 +198,13 @@ def test_grads_match_fd_open_ends(variant):
     for name in ("iso_a", "iso_b"):
         fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
         say(f"  {variant} d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 1.5e-9  # observed <= 4.9e-10
     for name, seed in (("field", 6), ("series", 8)):
         v = np.random.default_rng(seed).standard_normal(g[name].size)
         fd = pr.fd_dir(name, v, 1e-6)
         ad = float(g[name] @ v)
         say(f"  {variant} <dL/d{name}, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
-        assert rel(ad, fd) < 1e-6
+        assert rel(ad, fd) < 3e-7  # observed <= 9.4e-8 (no NS inlet, field)
 
 
 def test_grads_match_oracle_fd(pbb_problem):
@@ -223,7 +226,7 @@ def test_grads_match_oracle_fd(pbb_problem):
         h = 1e-5 * v0
         fd = (oracle_loss(**{attr: v0 + h}) - oracle_loss(**{attr: v0 - h})) / (2 * h)
         say(f"  oracle fd d L/d {name}: adjoint {g[name]:.12e} oracle {fd:.12e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 1.5e-9  # observed 4.2e-10
     c = int(np.argmax(np.abs(g["field"])))
     f0 = np.asarray(cfg.field, np.float64)
     fp, fm = f0.copy(), f0.copy()
@@ -231,7 +234,7 @@ def test_grads_match_oracle_fd(pbb_problem):
     fm[c] -= 1e-6
     fd = (oracle_loss(field=fp) - oracle_loss(field=fm)) / 2e-6
     say(f"  oracle fd d L/d field[{c}]: adjoint {g['field'][c]:.12e} oracle {fd:.12e}")
-    assert rel(g["field"][c], fd) < 1e-5
+    assert rel(g["field"][c], fd) < 6e-9  # observed 2.0e-9
 
 
 def test_fp32_adjoint_matches_fp64(pbb_problem):
@@ -240,11 +243,11 @@ def test_fp32_adjoint_matches_fp64(pbb_problem):
     g32 = p32.grads()
     for name in ("iso_a", "iso_b"):
         say(f"  f32 vs f64 d L/d {name}: {g32[name]:.8e} vs {g64[name]:.8e} rel {rel(g32[name], g64[name]):.2e}")
-        assert rel(g32[name], g64[name]) < 1e-3
+        assert rel(g32[name], g64[name]) < 1.5e-5  # observed 6.2e-6
     for name in ("field", "series"):
         e = np.linalg.norm(g32[name] - g64[name]) / np.linalg.norm(g64[name])
         say(f"  f32 vs f64 {name}: relative l2 {e:.2e}")
-        assert e < 1e-3
+        assert e < 2e-5  # observed 8.8e-6 (field), 6.2e-6 (series)
 
 
 @pytest.mark.parametrize("k", [1, 7, NSTEPS])
@@ -457,7 +460,7 @@ def test_c4_differentiation_fp64_vs_fd():
     for name in ("iso_a", "iso_b"):
         fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
         say(f"    d L/d {name}: adjoint {g[name]:.10e} fd {fd:.10e} rel {rel(g[name], fd):.2e}")
-        assert rel(g[name], fd) < 1e-6
+        assert rel(g[name], fd) < 1e-7  # observed 3.1e-8
     f0 = cfg.field
     # cells at least 1e-3 inside (0, 1), so that no perturbation crosses the clamp at 0 or 1
     # (C4's sphere cells lie in (0.95, 1]; a step of a few 1e-6 would push some across 1)
@@ -468,9 +471,9 @@ def test_c4_differentiation_fp64_vs_fd():
     ad = float(g["field"] @ v)
     say(f"    <dL/dfield, v> (1e-3 < field < 1 - 1e-3): adjoint {ad:.10e} fd(h) {fd:.10e} fd(4h) {fd4:.10e} "
         f"rel {rel(ad, fd):.2e}")
-    # central differences at h = 1e-6 carry ~5e-7 truncation here (their h / 4h difference), so the
-    # bound is 1e-5 (observed 2e-6)
-    assert rel(ad, fd) < 1e-5
+    # central differences at h = 1e-6 carry ~1e-6 truncation here (their h / 4h difference), so the
+    # bound is 5e-6 (observed 2.2e-6)
+    assert rel(ad, fd) < 5e-6
     # fluid cells sit at field = 0, where the gradient passed through is the one-sided (upward) one
     # (forward differences at h and 2h, extrapolated: 2 D(h) - D(2h) cancels the O(h) term, which the
     # drag of partial bounce-back in every fluid cell makes large)
@@ -483,7 +486,7 @@ def test_c4_differentiation_fp64_vs_fd():
     ad1 = float(g["field"] @ w)
     say(f"    <dL/dfield, w> (field = 0, upward): adjoint {ad1:.10e} one-sided fd {fd1:.10e} (D(h) {d1:.6e}) "
         f"rel {rel(ad1, fd1):.2e}")
-    assert rel(ad1, fd1) < 1e-4
+    assert rel(ad1, fd1) < 5e-6  # observed 1.8e-6
 
 
 def test_c4_fp32_adjoint_vs_fp64_adjoint():
@@ -499,17 +502,18 @@ def test_c4_fp32_adjoint_vs_fp64_adjoint():
     g32 = p32.grads()
     for name in ("iso_a", "iso_b"):
         say(f"  C4 f32 vs f64 d L/d {name}: {g32[name]:.8e} vs {g64[name]:.8e} rel {rel(g32[name], g64[name]):.2e}")
-        assert rel(g32[name], g64[name]) < 1e-2
+        assert rel(g32[name], g64[name]) < 4e-3  # observed 1.96e-3: the FP32 forward run's own drift
     e = np.linalg.norm(g32["field"] - g64["field"]) / np.linalg.norm(g64["field"])
     say(f"  C4 f32 vs f64 field gradient: relative l2 {e:.2e}")
-    assert e < 1e-2
+    assert e < 4e-3  # observed 1.92e-3
 
 
 def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
 
-@pytest.mark.parametrize("prec,bound_iso,bound_vec", [("f16", 1e-2, 5e-2), ("bf16", 5e-2, 2e-1)])
+# observed: FP16 1.04e-3 .. 1.32e-3, BF16 1.19e-2 .. 1.56e-2
+@pytest.mark.parametrize("prec,bound_iso,bound_vec", [("f16", 3e-3, 3e-3), ("bf16", 3e-2, 3.5e-2)])
 def test_half_state_adjoint_against_fp64(pbb_problem, prec, bound_iso, bound_vec):
     """16-bit deviation-stored checkpoints and segment states (FP32 arithmetic in the recompute and
     the adjoint) against the FP64 adjoint; the errors are reported."""
@@ -552,7 +556,8 @@ def test_half_checkpoints_of_fp32_run(pbb_problem, ckpt, k):
     e64.update({name: rel_l2(gh[name], g64[name]) for name in ("field", "series")})
     say(f"  f32 run, {ckpt} checkpoints (k={k}) vs f32 checkpoints:", {n: f"{v:.2e}" for n, v in errs.items()})
     say(f"  f32 run, {ckpt} checkpoints (k={k}) vs f64 adjoint:", {n: f"{v:.2e}" for n, v in e64.items()})
-    bound = 1e-3 if ckpt == "bf16" or k == 1 else 1e-4
+    # observed: FP16 k = 64 6.8e-6, BF16 k = 64 5.8e-5, FP16 k = 1 1.3e-4
+    bound = 3e-4 if k == 1 else (1.5e-4 if ckpt == "bf16" else 2e-5)
     assert max(errs.values()) < bound, errs
 
 
@@ -585,8 +590,9 @@ def test_c5_half_checkpoints_against_fp64():
                 iso_a=rel(g16["iso_a"], g64["iso_a"]), iso_b=rel(g16["iso_b"], g64["iso_b"]))
     say(f"  C5 f32 run + f16 checkpoints vs f64 adjoint: {({k: f'{v:.2e}' for k, v in errs.items()})}, "
         f"|dL/dknots| {np.linalg.norm(k64):.3e}")
-    assert errs["c_out"] < 1e-3 and errs["knots"] < 3e-3 and errs["field"] < 3e-3, errs
-    assert errs["iso_a"] < 3e-3 and errs["iso_b"] < 3e-3, errs
+    # observed: c_out 3.6e-5, knots 2.7e-4, field 2.8e-4, iso 2.6e-4
+    assert errs["c_out"] < 1e-4 and errs["knots"] < 6e-4 and errs["field"] < 6e-4, errs
+    assert errs["iso_a"] < 6e-4 and errs["iso_b"] < 6e-4, errs
 
 
 def test_c5_full_size_fp16_checkpoints_against_fp64():

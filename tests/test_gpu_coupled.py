@@ -144,7 +144,7 @@ def test_pair_kernel_equals_single_cell_kernel(prec, variant, monkeypatch):
     else:
         errs = {k: rel_err(host(out[0][k]), host(out[1][k])) for k in ("phi", "q", "c_out")}
         say("  pair vs single:", {k: f"{v:.2e}" for k, v in errs.items()})
-        assert all(v < 5e-3 for v in errs.values()), errs
+        assert all(v < 1e-3 for v in errs.values()), errs  # observed <= 3.5e-4
 
 
 @pytest.mark.parametrize("prec", ["f32", "f16"])
@@ -238,7 +238,7 @@ def test_c1_fp32_against_fp64_oracle():
     errs = dict(rho=rel_err(rho, orc.get(R.RHO)), ux=rel_err(ux, orc.get(R.UX)), phi=rel_err(host(sim.phi), orc.get(R.PHI)),
                 q=rel_err(host(sim.q), orc.get(R.Q)), c_out=rel_err(c, cr))
     say("  f32 vs f64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert all(v < 1e-4 for v in errs.values()), errs
+    assert all(v < 8e-5 for v in errs.values()), errs  # observed <= 3.9e-5 (u)
 
 
 @pytest.fixture(scope="module")
@@ -252,7 +252,8 @@ def breakthrough_ref():
     return cfg, ref
 
 
-@pytest.mark.parametrize("prec,bound", [("f32", 5e-5), ("f16", 1e-3), ("bf16", 5e-3)])
+# observed breakthrough errors: FP32 1.6e-5, FP16 1.9e-4, BF16 1.2e-3
+@pytest.mark.parametrize("prec,bound", [("f32", 3.5e-5), ("f16", 4e-4), ("bf16", 2.5e-3)])
 def test_storage_precision_against_fp64(breakthrough_ref, prec, bound):
     """Population storage precision against the FP64 oracle over a breakthrough (errors reported per field;
     the breakthrough error is max_t |dC_out(t)| / max_t C_out(t))."""

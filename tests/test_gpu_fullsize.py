@@ -113,10 +113,11 @@ def test_c3_fp32_against_fp64(c3):
     errs = {k: float((out["f32"][k] - out["f64"][k]).abs().max() / out["f64"][k].abs().max().clamp_min(1e-300))
             for k in out["f64"]}
     say("  f32 vs f64:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert all(v < 5e-5 for v in errs.values()), errs
+    assert all(v < 2.5e-5 for v in errs.values()), errs  # observed <= 1.1e-5 (u)
 
 
-@pytest.mark.parametrize("prec,bound", [("f16", 1e-3), ("bf16", 5e-3)])
+# observed: FP16 <= 4.3e-4 (u), BF16 <= 3.1e-3 (u)
+@pytest.mark.parametrize("prec,bound", [("f16", 9e-4), ("bf16", 5e-3)])
 def test_c3_half_storage_against_fp32(c3, prec, bound):
     """512^3, 100 steps from the equilibrium start: 16-bit deviation storage against FP32 storage,
     relative to each field's scale; the outlet curve, still at the 1e-8 concentration floor after 100
