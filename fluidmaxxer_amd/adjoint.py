@@ -42,8 +42,6 @@ class AdjointRun:
         ckpt = prec if ckpt is None else ckpt
         if ckpt != prec and not (prec == "f32" and ckpt in ("f16", "bf16")):
             raise abi.FmxError(f"checkpoints of a {prec} run are stored in {prec} (or f16 / bf16 for f32)")
-        if cfg.ns_inlet == "neq":
-            raise abi.FmxError("the NEQ velocity inlet has no adjoint; use ns_inlet='eq'")
         if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):
             raise abi.FmxError("inlet_series must hold steps 0..nsteps")
         self.cfg, self.nsteps, self.k = cfg, int(nsteps), int(k)

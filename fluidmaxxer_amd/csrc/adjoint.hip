@@ -18,6 +18,10 @@
 // Boundary rules (the boundary conditions as the fused step folds them in):
 //   * x = 0 with an equilibrium NS inlet, or an ADDirichlet AD inlet: the stored values are
 //     constants, so the owner's adjoint of them is zero.
+//   * x = 0 with the NEQ velocity inlet: the stored values are the next step's
+//     feq(r2, u_BC) + f - feq(r2, u2) of the (1, y, z) cell's post-collision populations f and their
+//     moments r2, u2 (VelocityDirichletNEQ.cu:81-95), so that cell's reverse step gathers their
+//     adjoint and back-propagates it; the x = 0 cell's own collision result is replaced and gets none.
 //   * x = nx-1 with a pressure outlet / Neumann outlet: the outlet slots hold functions of the
 //     (nx-2, y, z) cell's post-collision state (the pressure equilibrium of its velocity, a copy
 //     of its g), so that cell's reverse step gathers their adjoint and back-propagates it; the
@@ -316,6 +320,35 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
             });
         }
         C lux = C(0), luy = C(0), luz = C(0);  // adjoint of the post-collision velocity
+        if (prm.ns_inlet == 1 && x == 1) {
+            // the NEQ inlet values of (0, y, z): fB = feq(r2, u_BC) + f - feq(r2, u2) with f, r2, u2 this
+            // cell's post-collision populations and moments.  dfB/df is the identity; through r2 (u2
+            // held) every fB gains w (P(u_BC) - P(u2)), P(u) = 1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2; through
+            // u2, -r2 w (3 e + 9 (e.u2) e - 3 u2); u2 = v / r2 goes back with the coupling below.
+            const std::uint32_t lo = ld_b(p.links, (cell - 1u) * 4u);
+            const C u2N = C(1.5) * (ux2 * ux2 + uy2 * uy2 + uz2 * uz2), u2B = C(1.5) * prm.uin2;
+            C lr2 = C(0), sw = C(0), tx = C(0), ty = C(0), tz = C(0);
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const C lb = gf_local(qc, 0, cell - 1u, lo, true, gf_main(qc, 0, cell - 1u, lo, true));
+                lfr[q] += lb;
+                const C w = w19<C>(q);
+                const C euN = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux2, uy2, uz2);
+                const C euB = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(prm.uin_x, prm.uin_y, prm.uin_z);
+                const C cuN = C(3) * euN, cuB = C(3) * euB;
+                lr2 += lb * w * ((cuB + C(0.5) * cuB * cuB - u2B) - (cuN + C(0.5) * cuN * cuN - u2N));
+                const C t = lb * w * C(3) * (C(1) + cuN);
+                tx = axpy1<Q19::cx[q]>(tx, t);
+                ty = axpy1<Q19::cy[q]>(ty, t);
+                tz = axpy1<Q19::cz[q]>(tz, t);
+                sw += lb * w;
+            });
+            lux -= r2 * (tx - C(3) * ux2 * sw);
+            luy -= r2 * (ty - C(3) * uy2 * sw);
+            luz -= r2 * (tz - C(3) * uz2 * sw);
+#pragma unroll
+            for (int q = 0; q < 19; ++q) lfr[q] += lr2;
+        }
         if (writes_outlet_f) {  // the outlet cell's populations feq(rho_out, u2) (PressureDirichlet.cu:81-95)
             const std::uint32_t lo = ld_b(p.links, (cell + 1u) * 4u);
             C sw = C(0);
@@ -739,9 +772,6 @@ static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
                          "precision, or FP16 / BF16 for an FP32 run)",
                          who, cp, d->prec);
 
-    if (d->ns_inlet == 1)
-        return set_error(FMX_ERR_UNSUPPORTED, "%s: the NEQ velocity inlet has no adjoint (use the equilibrium inlet)",
-                         who);
     if (step0 < 0 || nsteps < 1 || k < 1) return set_error(FMX_ERR_ARG, "%s: bad step range or interval", who);
     return check_run_buffers(*d, *b, step0, nsteps, who, writes_outlet);
 }

@@ -247,7 +247,7 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
 
 /* ------------------------------------------------------ discrete adjoint */
 /* No counterpart in the reference (it has no adjoint): the exact reverse-mode derivative of the
- * fused step above (fmx_coupled_run), equilibrium or no NS inlet.  With FP16 / BF16 storage the
+ * fused step above (fmx_coupled_run), with any of its NS inlets (none, NEQ, equilibrium).  With FP16 / BF16 storage the
  * checkpoints and segment states keep the 16-bit deviation encoding and the reverse steps run in FP32.
  *
  * Loss: any L whose dependence on the run is through the outlet means C_out(s) (fmx_outlet_mean);

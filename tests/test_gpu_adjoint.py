@@ -55,6 +55,8 @@ def adj_config(op="pbb", series=True, closed=False, seed=21, dims=(16, 10, 10), 
     elif variant == "no_outlets":    # keep-local pulls at x = nx-1 for both lattices
         cfg.ns_outlet = False
         cfg.ad_outlet = False
+    elif variant == "neq_inlet":     # main.cpp's VelocityDirichletNEQ inlet
+        cfg.ns_inlet = "neq"
     if closed:
         cfg.ns_inlet = None
         cfg.ns_outlet = False
@@ -205,6 +207,31 @@ def test_grads_match_fd_open_ends(variant):
         ad = float(g[name] @ v)
         say(f"  {variant} <dL/d{name}, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
         assert rel(ad, fd) < 3e-7  # observed <= 9.4e-8 (no NS inlet, field)
+
+
+@pytest.mark.parametrize("op", ["pbb", "bgk"])
+def test_grads_match_fd_neq_inlet(op):
+    """main.cpp's VelocityDirichletNEQ inlet (VelocityDirichletNEQ.cu:81-95): the x = 0 populations are
+    functions of the (1, y, z) cell's post-collision state, so their adjoint flows back into it
+    (through f, its density and its velocity).  FP64 adjoint against FP64 central differences, for the
+    isotherm constants and random directions in the field and the inlet series; and the gradients are
+    not those of the equilibrium inlet (the NEQ terms are live)."""
+    pr = Problem(adj_config(op, variant="neq_inlet"))
+    g = pr.grads()
+    for name in ("iso_a", "iso_b"):
+        fd = pr.fd_scalar(name, 1e-5 * pr.base[name])
+        say(f"  neq {op} d L/d {name}: adjoint {g[name]:.12e} fd {fd:.12e} rel {rel(g[name], fd):.2e}")
+        assert rel(g[name], fd) < 5e-10  # observed <= 1.1e-10
+    for name, seed in (("field", 6), ("series", 8)):
+        if g[name] is None:
+            continue
+        v = np.random.default_rng(seed).standard_normal(g[name].size)
+        fd = pr.fd_dir(name, v, 1e-6)
+        ad = float(g[name] @ v)
+        say(f"  neq {op} <dL/d{name}, v>: adjoint {ad:.12e} fd {fd:.12e} rel {rel(ad, fd):.2e}")
+        assert rel(ad, fd) < 3e-8  # observed <= 8.4e-9
+    eq = Problem(adj_config(op), target=pr.target).grads()
+    assert rel(eq["iso_a"], g["iso_a"]) > 1e-4
 
 
 def test_grads_match_oracle_fd(pbb_problem):
