@@ -13,7 +13,8 @@ reference update) timed serially on a bounded sample of the same workload.
 Extra keys: the same box with FP16 / BF16 storage, the 256x128x128 pack (C2),
 and the adjoint/forward time ratio of the checkpointed gradient on the
 differentiation workload (C4, 128x64x64 partial bounce-back, 4000 steps,
-checkpoint every 200).
+checkpoint every 200): headed by the FP64 run with FP32 checkpoints and an FP32
+reverse sweep, with the all-FP32 and all-FP64 gradients beside it.
 """
 from __future__ import annotations
 
@@ -142,13 +143,13 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None, ckpt=None, two_
     import torch
 
     from fluidmaxxer_amd import abi, workloads
-    from fluidmaxxer_amd.adjoint import AdjointRun
+    from fluidmaxxer_amd.adjoint import AdjointRun, segment_slots
 
     nsteps = steps or workloads.SPECS[workload]["steps"]
     cfg = workloads.build(workload, steps=nsteps)
     seg_slots = None
     if two_level:
-        seg_slots = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+        seg_slots = segment_slots(k, prec, ckpt, two_level=True)
     run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt, seg_slots=seg_slots)
     sim = run.sim
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
@@ -179,6 +180,8 @@ def adjoint_timing(workload="C4", prec="f32", k=200, steps=None, ckpt=None, two_
     res = {
         "workload": f"{workload}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, {nsteps} steps, "
                     f"checkpoint every {k} ({ckpt or prec}), {prec} run, {run.seg_slots} segment slots",
+        "run_precision": prec,
+        "sweep_precision": "f64" if (prec == "f64" and ckpt in (None, "f64")) else "f32",
         "store_gib": round((run.ckpt.numel() + run.seg.numel()) / 2 ** 30, 2),
         "forward_ms": round(t_plain, 3),
         "forward_with_checkpoints_ms": round(t_fwd, 3),
@@ -202,7 +205,7 @@ def adjoint_c5_accuracy(steps=None, k=500):
     import torch
 
     from fluidmaxxer_amd import abi, workloads
-    from fluidmaxxer_amd.adjoint import AdjointRun
+    from fluidmaxxer_amd.adjoint import AdjointRun, segment_slots
 
     nsteps = steps or workloads.SPECS["C5"]["steps"]
     cfg = workloads.build("C5", steps=nsteps)
@@ -217,7 +220,7 @@ def adjoint_c5_accuracy(steps=None, k=500):
     B[np.arange(nsteps + 1), i0 + 1] = s - i0
 
     def gradients(prec, ckpt, target_of):
-        seg = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+        seg = segment_slots(k, prec, ckpt, two_level=True)
         run = AdjointRun(cfg, nsteps, k, prec, ckpt=ckpt, seg_slots=seg)
         c = run.forward().cpu().numpy()
         target = target_of(c)
@@ -241,6 +244,7 @@ def adjoint_c5_accuracy(steps=None, k=500):
     g64 = gradients("f64", None, lambda c: 0.85 * c)
     g16 = gradients("f32", "f16", lambda c: g64["target"])
     g32 = gradients("f32", None, lambda c: g64["target"])
+    gm = gradients("f64", "f32", lambda c: g64["target"])
     return {
         "workload": f"C5: {cfg.nx}x{cfg.ny}x{cfg.nz} pbb langmuir, {nsteps} steps; FP32 run, FP16 checkpoints every {k}, "
                     f"FP32 recompute, against the FP64 adjoint (two-level stores)",
@@ -248,8 +252,9 @@ def adjoint_c5_accuracy(steps=None, k=500):
         "rel_l2": errors(g16, g64),
         "rel_l2_fp32_ckpt_vs_fp64": errors(g32, g64),
         "rel_l2_fp16_ckpt_vs_fp32_ckpt": errors(g16, g32),
+        "rel_l2_fp64_run_fp32_sweep_vs_fp64": errors(gm, g64),
         "backward_seconds": {"f64": round(g64["seconds"], 2), "f32_f16ckpt": round(g16["seconds"], 2),
-                             "f32": round(g32["seconds"], 2)},
+                             "f32": round(g32["seconds"], 2), "f64_run_f32_sweep": round(gm["seconds"], 2)},
     }
 
 
@@ -380,7 +385,8 @@ def main():
     ap.add_argument("--stream-probe", action="store_true",
                     help="instead of the benchmark, measure plain streaming bandwidth (copy / read / write)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing on C4")
-    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward timing (C4)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint/forward timings on C4 (FP64 run with FP32 sweep, FP32, FP64)")
     ap.add_argument("--no-storage-sweep", "--no-large-box", dest="no_storage_sweep", action="store_true",
                     help="skip the FP16 / BF16 storage timings of the same workload")
     ap.add_argument("--no-c2", action="store_true", help="skip the 256x128x128 (C2) FP32 timing")
@@ -424,11 +430,14 @@ def main():
         for two_level in (False, True):
             print(json.dumps({"adjoint": adjoint_timing("C5", "f32", k=500, steps=args.adjoint_steps, ckpt="f16",
                                                         two_level=two_level)}), flush=True)
+        # the FP64 run with FP32 checkpoints and an FP32 reverse sweep (two levels)
+        print(json.dumps({"adjoint": adjoint_timing("C5", "f64", k=500, steps=args.adjoint_steps, ckpt="f32",
+                                                    two_level=True)}), flush=True)
         return
 
     if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_timing(prec="f64" if args.prec == "f64" else "f32",
-                                                    steps=args.adjoint_steps)}), flush=True)
+        for prec, ckpt in (("f64", "f32"), ("f32", None), ("f64", None)):
+            print(json.dumps({"adjoint": adjoint_timing(prec=prec, ckpt=ckpt, steps=args.adjoint_steps)}), flush=True)
         return
 
     if args.calibrate_copy:
@@ -523,7 +532,10 @@ def main():
         line["c2"] = storage_timing(c2, ("f32",), steps=max(steps, 100), warmup=20)
         del c2
     if not args.no_adjoint and args.prec in ("f32", "f64"):
-        line["adjoint"] = adjoint_timing(prec=args.prec)
+        # the FP64 run with an FP32 reverse sweep: FP64 forward curve, gradients within 1e-5 of the FP64
+        # adjoint; ratio over the FP64 forward run
+        line["adjoint"] = adjoint_timing(prec="f64", ckpt="f32")
+        line["adjoint_single_precision"] = {p: adjoint_timing(prec=p) for p in ("f32", "f64")}
     if not args.no_cpu_baseline:
         line["cpu_baseline"], all_cores = cpu_baseline(cfg, args.prec, args.cpu_seconds)
         if all_cores is not None:

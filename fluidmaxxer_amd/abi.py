@@ -22,6 +22,7 @@ ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
 MASS_COLS = 4
+ABI_VERSION = 3  # FMX_ABI_VERSION of the header these structs mirror
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
@@ -71,6 +72,7 @@ class AdjointBufs(C.Structure):
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("outlet_weight", C.c_void_p),
         ("outlet_weight_len", C.c_int64),
         ("grad_iso", C.c_void_p), ("grad_field", C.c_void_p), ("grad_series", C.c_void_p),
+        ("field_rev", C.c_void_p), ("tau_field_rev", C.c_void_p),
     ]
 
 
@@ -175,6 +177,11 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = res
+        # the structs above mirror one layout of include/fmx_abi.h: a library built from another
+        # header would read them wrongly
+        if L.fmx_abi_version() != ABI_VERSION:
+            raise FmxError(f"{LIB_PATH} implements ABI version {L.fmx_abi_version()}, these bindings version "
+                           f"{ABI_VERSION}: rebuild it with `python -m fluidmaxxer_amd.build`")
         _lib = L
     return _lib
 

@@ -5,7 +5,10 @@ adjoint buffers.  `forward()` runs the configuration from its initial state for
 `nsteps` steps (fmx_adjoint_forward, a checkpoint every `k` steps) and returns the
 outlet-averaged concentration C_out(s), s = 1..nsteps.  Checkpoints keep the storage
 precision, or, for an FP32 run, `ckpt="f16"` / `"bf16"` stores them as 16-bit
-deviations and the reverse sweep recomputes each segment in FP32 from them.  `backward(w)` runs the
+deviations and the reverse sweep recomputes each segment in FP32 from them, or, for an
+FP64 run, `ckpt="f32"` rounds them to FP32 and the whole reverse sweep runs in FP32 (the
+forward curve stays the FP64 run's; the gradients are the FP32 update's along the FP64
+trajectory, restarted from it at every checkpoint).  `backward(w)` runs the
 reverse sweep (fmx_adjoint_backward) for a loss with dL/dC_out(s) = w[s-1] and
 returns the gradients with respect to the isotherm constants (Langmuir q_max and
 K_L, or the linear K_Henry), the per-cell field (partial-bounce-back solid fraction
@@ -33,6 +36,17 @@ def _torch():
     return torch
 
 
+def decoded_checkpoints(prec: str, ckpt: str = None) -> bool:
+    """True for 16-bit checkpoints of an FP32 run, which the reverse sweep decodes into a segment slot
+    (the `recast` argument of fmx_adjoint_seg_slots)."""
+    return prec == "f32" and ckpt not in (None, "f32")
+
+
+def segment_slots(k: int, prec: str, ckpt: str = None, two_level: bool = False) -> int:
+    """Segment slots a reverse sweep over segments of k steps needs (fmx_adjoint_seg_slots)."""
+    return int(abi.lib().fmx_adjoint_seg_slots(int(k), int(decoded_checkpoints(prec, ckpt)), int(two_level)))
+
+
 class AdjointRun:
     def __init__(self, cfg: CoupledConfig, nsteps: int, k: int, prec: str = "f64", device: str = "cuda:0",
                  ckpt: str = None, seg_slots: int = None):
@@ -40,14 +54,19 @@ class AdjointRun:
         if prec not in ("f32", "f64", "f16", "bf16"):
             raise abi.FmxError(f"unknown precision {prec}")
         ckpt = prec if ckpt is None else ckpt
-        if ckpt != prec and not (prec == "f32" and ckpt in ("f16", "bf16")):
-            raise abi.FmxError(f"checkpoints of a {prec} run are stored in {prec} (or f16 / bf16 for f32)")
+        if ckpt != prec and not ((prec == "f32" and ckpt in ("f16", "bf16")) or (prec == "f64" and ckpt == "f32")):
+            raise abi.FmxError(f"checkpoints of a {prec} run are stored in {prec} (or f16 / bf16 for f32, f32 for "
+                               f"f64)")
         if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):
             raise abi.FmxError("inlet_series must hold steps 0..nsteps")
         self.cfg, self.nsteps, self.k = cfg, int(nsteps), int(k)
         self.sim = CoupledSim(cfg, prec=prec, device=device)
         self.device = self.sim.device
         self.cdt = self.sim.cdt
+        # an FP64 run with FP32 checkpoints sweeps back in FP32: its adjoint state and segment slots
+        # are FP32, and so are the copies of the per-cell fields the FP32 kernels read
+        self.mixed = prec == "f64" and ckpt == "f32"
+        self.rdt = torch.float32 if self.mixed else self.cdt
         n = cfg.n
         d = self.sim._desc
         self.state_bytes = int(abi.lib().fmx_adjoint_state_bytes(abi.C.byref(d)))
@@ -55,21 +74,22 @@ class AdjointRun:
         self.ckpt_bytes = int(abi.lib().fmx_adjoint_ckpt_bytes(abi.C.byref(d), self.ckpt_prec))
         nseg = (self.nsteps + self.k - 1) // self.k
         kk = min(self.k, self.nsteps)
-        recast = int(ckpt != prec)
         # segment store: one level by default; fewer slots (at least the two-level minimum, about
         # 2 sqrt(k)) make the reverse sweep keep sub-checkpoints within each segment
-        single = int(abi.lib().fmx_adjoint_seg_slots(kk, recast, 0))
+        single = segment_slots(kk, prec, ckpt)
         self.seg_slots = single if seg_slots is None else int(seg_slots)
-        least = int(abi.lib().fmx_adjoint_seg_slots(kk, recast, 1))
+        least = segment_slots(kk, prec, ckpt, two_level=True)
         if self.seg_slots < least:
             raise abi.FmxError(f"segments of {kk} steps need at least {least} segment slots")
         u8 = torch.uint8
         self.ckpt = torch.empty(nseg * self.ckpt_bytes, dtype=u8, device=self.device)
-        self.seg = torch.empty(max(self.seg_slots, 1) * self.state_bytes, dtype=u8, device=self.device)
-        self.lam_f = [torch.empty(19 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
-        self.lam_g = [torch.empty(7 * n, dtype=self.cdt, device=self.device) for _ in range(2)]
-        self.lam_q = torch.empty(n, dtype=self.cdt, device=self.device)
-        self.lam_phi = torch.empty(n, dtype=self.cdt, device=self.device)
+        slot_bytes = self.ckpt_bytes if self.mixed else self.state_bytes
+        self.seg = torch.empty(max(self.seg_slots, 1) * slot_bytes, dtype=u8, device=self.device)
+        self.lam_f = [torch.empty(19 * n, dtype=self.rdt, device=self.device) for _ in range(2)]
+        self.lam_g = [torch.empty(7 * n, dtype=self.rdt, device=self.device) for _ in range(2)]
+        self.lam_q = torch.empty(n, dtype=self.rdt, device=self.device)
+        self.lam_phi = torch.empty(n, dtype=self.rdt, device=self.device)
+        self.field_rev = self.tau_field_rev = None  # FP32 copies, made at backward()
         self.workspace = None  # sized at backward() (it follows the launch shape)
         self.grad_iso = torch.zeros(2, dtype=torch.float64, device=self.device)
         self.grad_field = torch.zeros(n, dtype=torch.float64, device=self.device) if self.sim.field is not None else None
@@ -149,6 +169,13 @@ class AdjointRun:
         a.grad_iso = self.grad_iso.data_ptr()
         a.grad_field = self.grad_field.data_ptr() if self.grad_field is not None else None
         a.grad_series = self.grad_series.data_ptr() if self.grad_series is not None else None
+        if self.mixed:
+            if sim.field is not None:
+                self.field_rev = sim.field.to(torch.float32)
+                a.field_rev = self.field_rev.data_ptr()
+            if sim.tau_field is not None:
+                self.tau_field_rev = sim.tau_field.to(torch.float32)
+                a.tau_field_rev = self.tau_field_rev.data_ptr()
         abi.call("fmx_adjoint_backward", sim._desc_p(), sim._bufs_p(), 1, self.nsteps, self.k, self.ckpt.data_ptr(),
                  self.ckpt_prec, self.seg.data_ptr(), self.seg_slots, abi.C.byref(a), sim._s())
         sim.synchronize()

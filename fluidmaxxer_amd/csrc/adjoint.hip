@@ -583,8 +583,22 @@ __global__ void __launch_bounds__(256) k_ckpt_decode(const typename Storage<CP>:
     for (int q = 0; q < 7; ++q) g[i64(q) * n + c] = w7<float>(q) * (ph + widen(hg[i64(q) * n + c]));
 }
 
+// Element-wise precision change of one state section (FP32 checkpoints of an FP64 run).
+template <class From, class To>
+__global__ void __launch_bounds__(256) k_cast(const From *__restrict__ src, To *__restrict__ dst, i64 count) {
+    for (i64 i = gtid(); i < count; i += i64(gridDim.x) * blockDim.x) dst[i] = To(src[i]);
+}
+
 // ------------------------------------------------------------- host side
 inline i64 align256(i64 b) { return (b + 255) & ~i64(255); }
+
+template <class From, class To> static int cast_section(const void *src, void *dst, i64 count, hipStream_t s) {
+    const i64 want = (count + 255) / 256;
+    const unsigned grid = unsigned(want < 1 ? 1 : (want > 65536 ? 65536 : want));
+    hipLaunchKernelGGL((k_cast<From, To>), dim3(grid), dim3(256), 0, s, static_cast<const From *>(src),
+                       static_cast<To *>(dst), count);
+    return after_launch("k_cast");
+}
 
 // Reverse steps whose per-block inlet-series sums the workspace holds before they are added up.
 constexpr int kSeriesRows = 64;
@@ -675,13 +689,32 @@ static int load_half_ckpt(const fmx_coupled_desc &d, const StateView<CP> &src, c
     return check_hip(hipMemcpyAsync(dst.phi, src.phi, n * sizeof(float), hipMemcpyDeviceToDevice, s), "load phi");
 }
 
-// The state of b into checkpoint slot j: as stored, or (FP32 run, 16-bit checkpoints) encoded.
+// The current FP64 state of b rounded into an FP32 checkpoint slot, and such a slot widened back.
+static int save_narrow_ckpt(const fmx_coupled_bufs &b, int cur, const StateView<FMX_F32> &dst, i64 n, hipStream_t s) {
+    if (int rc = cast_section<double, float>(b.f[cur], dst.f, 19 * n, s)) return rc;
+    if (int rc = cast_section<double, float>(b.g[cur], dst.g, 7 * n, s)) return rc;
+    if (int rc = cast_section<double, float>(b.q, dst.q, n, s)) return rc;
+    return cast_section<double, float>(b.phi[cur], dst.phi, n, s);
+}
+
+static int load_narrow_ckpt(const fmx_coupled_bufs &b, int cur, const StateView<FMX_F32> &src, i64 n, hipStream_t s) {
+    if (int rc = cast_section<float, double>(src.f, b.f[cur], 19 * n, s)) return rc;
+    if (int rc = cast_section<float, double>(src.g, b.g[cur], 7 * n, s)) return rc;
+    if (int rc = cast_section<float, double>(src.q, b.q, n, s)) return rc;
+    return cast_section<float, double>(src.phi, b.phi[cur], n, s);
+}
+
+// The state of b into checkpoint slot j: as stored, (FP32 run, 16-bit checkpoints) encoded, or (FP64
+// run, FP32 checkpoints) rounded.
 template <int P>
 static int save_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, void *ckpt, int cp, i64 j, i64 n,
                      hipStream_t s) {
     if constexpr (P == FMX_F32) {
         if (cp == FMX_F16) return save_half_ckpt<FMX_F16>(d, b, cur, state_at<FMX_F16>(ckpt, j, n), n, s);
         if (cp == FMX_BF16) return save_half_ckpt<FMX_BF16>(d, b, cur, state_at<FMX_BF16>(ckpt, j, n), n, s);
+    }
+    if constexpr (P == FMX_F64) {
+        if (cp == FMX_F32) return save_narrow_ckpt(b, cur, state_at<FMX_F32>(ckpt, j, n), n, s);
     }
     return save_state<P>(b, cur, state_at<P>(ckpt, j, n), n, s);
 }
@@ -760,8 +793,13 @@ static int adjoint_step(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, co
     return after_launch("k_adjoint");
 }
 
-// Checkpoints at the storage precision, or 16-bit checkpoints of an FP32 run.
-static bool ckpt_prec_ok(int prec, int cp) { return cp == prec || (prec == FMX_F32 && (cp == FMX_F16 || cp == FMX_BF16)); }
+// Checkpoints at the storage precision, 16-bit checkpoints of an FP32 run, or FP32 checkpoints of an
+// FP64 run (whose reverse sweep then runs in FP32).
+static bool ckpt_prec_ok(int prec, int cp) {
+    return cp == prec || (prec == FMX_F32 && (cp == FMX_F16 || cp == FMX_BF16)) || (prec == FMX_F64 && cp == FMX_F32);
+}
+// 16-bit checkpoints of an FP32 run: decoded into a segment slot before the segment is recomputed
+static bool decoded_ckpt(int prec, int cp) { return prec == FMX_F32 && cp != FMX_F32; }
 
 static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int64_t step0, int32_t nsteps,
                          int32_t k, int cp, const char *who, bool writes_outlet = false) {
@@ -769,7 +807,7 @@ static int adjoint_check(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
     if (!ckpt_prec_ok(d->prec, cp))
         return set_error(FMX_ERR_UNSUPPORTED,
                          "%s: checkpoint precision %d with storage precision %d (checkpoints keep the storage "
-                         "precision, or FP16 / BF16 for an FP32 run)",
+                         "precision, or FP16 / BF16 for an FP32 run, or FP32 for an FP64 run)",
                          who, cp, d->prec);
 
     if (step0 < 0 || nsteps < 1 || k < 1) return set_error(FMX_ERR_ARG, "%s: bad step range or interval", who);
@@ -1043,6 +1081,8 @@ int fmx_checkpoint_load(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, in
         if (ckpt_prec == FMX_F16) return load_half_ckpt<FMX_F16>(*d, state_at<FMX_F16>(ck, slot, n), dst, n, s);
         return load_half_ckpt<FMX_BF16>(*d, state_at<FMX_BF16>(ck, slot, n), dst, n, s);
     }
+    if (d->prec == FMX_F64 && ckpt_prec == FMX_F32)
+        return load_narrow_ckpt(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
     switch (d->prec) {
         case FMX_F32: return restore_state<FMX_F32>(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
         case FMX_F64: return restore_state<FMX_F64>(*b, cur, state_at<FMX_F64>(ck, slot, n), n, s);
@@ -1083,7 +1123,7 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
                          const fmx_adjoint_bufs *adj, void *stream) {
     FMX_TRY_BEGIN
     if (int rc = adjoint_check(d, b, step0, nsteps, k, ckpt_prec, "fmx_adjoint_backward")) return rc;
-    const bool recast = ckpt_prec != d->prec;
+    const bool recast = decoded_ckpt(d->prec, ckpt_prec);
     const int kk = nsteps < k ? nsteps : k;
     const i64 need = seg_slots_min(kk, recast);
     if (!ckpt || !adj || (need > 0 && (!seg || seg_slots < need)))
@@ -1104,6 +1144,19 @@ int fmx_adjoint_backward(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
     if (int rc = check_weights(adj, nsteps, "fmx_adjoint_backward")) return rc;
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
+    if (d->prec == FMX_F64 && ckpt_prec == FMX_F32) {
+        // The FP32 sweep over the FP64 run's rounded checkpoints: the FP32 step and adjoint with the
+        // caller's FP32 copies of the per-cell fields (the FP64 run's are not readable as FP32).
+        if ((b->field && !adj->field_rev) || (b->tau_field && !adj->tau_field_rev))
+            return set_error(FMX_ERR_ARG, "fmx_adjoint_backward: FP32 checkpoints of an FP64 run need field_rev / "
+                                          "tau_field_rev, the FP32 copies of field / tau_field");
+        fmx_coupled_desc d32 = *d;
+        d32.prec = FMX_F32;
+        fmx_coupled_bufs b32 = *b;
+        b32.field = b->field ? adj->field_rev : nullptr;
+        b32.tau_field = b->tau_field ? adj->tau_field_rev : nullptr;
+        return backward<FMX_F32>(d32, b32, step0, nsteps, k, ck, FMX_F32, seg, seg_slots, *adj, s);
+    }
     switch (d->prec) {
         case FMX_F32: return backward<FMX_F32>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);
         case FMX_F64: return backward<FMX_F64>(*d, *b, step0, nsteps, k, ck, ckpt_prec, seg, seg_slots, *adj, s);

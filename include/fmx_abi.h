@@ -28,8 +28,10 @@ extern "C" {
 
 /* 2: buffer lengths in fmx_coupled_bufs (inlet_series_len, outlet_acc_rows) and fmx_adjoint_bufs
  * (outlet_weight_len), checked before any launch; fmx_reduce_mass with a caller workspace and a
- * non-finite count, and its periodic log inside fmx_coupled_run (mass_log) */
-#define FMX_ABI_VERSION 2
+ * non-finite count, and its periodic log inside fmx_coupled_run (mass_log).
+ * 3: FP32 checkpoints of an FP64 run, whose reverse sweep runs in FP32 (fmx_adjoint_bufs field_rev,
+ * tau_field_rev) */
+#define FMX_ABI_VERSION 3
 
 /* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
 enum { FMX_F32 = 0, FMX_F64 = 1, FMX_F16 = 2, FMX_BF16 = 3 };
@@ -263,6 +265,12 @@ int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, dou
  * for an FP32 run, FMX_F16 / FMX_BF16: f and g are checkpointed as 16-bit deviations (f/w - f_ref,
  * g/w - phi_src, the 16-bit storage layout of the forward step), q and phi in FP32, and each segment
  * is recomputed in FP32 from its decoded checkpoint (seg then holds k slots instead of k-1).
+ * For an FP64 run, FMX_F32: the FP64 state rounded to FP32 at each checkpoint, and the whole reverse
+ * sweep -- segment recompute and adjoint steps -- in FP32 (seg slots, lam_* and field_rev /
+ * tau_field_rev in FP32; seg slots of fmx_adjoint_ckpt_bytes(desc, FMX_F32) bytes, k-1 of them).  The
+ * forward curve is the FP64 run's; the gradients are those of the FP32 update along the FP64 run's
+ * trajectory, restarted from it at every checkpoint, so the FP32 run's own drift is confined to
+ * one segment.
  * Two levels: given fewer segment slots than that (at least fmx_adjoint_seg_slots(k, recast, 1),
  * about 2 sqrt(k)), the sweep keeps sub-checkpoints every m steps within a segment and recomputes
  * one sub-segment at a time (one more forward pass over each segment; the same gradients, bit for
@@ -280,12 +288,15 @@ typedef struct fmx_adjoint_bufs {
     double *grad_iso;            /* 2 doubles (written) */
     double *grad_field;          /* N doubles (written) or NULL */
     double *grad_series;         /* indexed like inlet_series; entries step0 .. step0+nsteps-1 written, or NULL */
+    const void *field_rev;       /* FP64 run with FP32 checkpoints: bufs->field in FP32 (N; NULL where field is) */
+    const void *tau_field_rev;   /* likewise bufs->tau_field in FP32; both unused otherwise */
 } fmx_adjoint_bufs;
 
 /* Bytes of one stored state (f, g, q, phi in the storage precision; each section 256-byte aligned) */
 int64_t fmx_adjoint_state_bytes(const fmx_coupled_desc *desc);
 /* Bytes of one checkpoint slot at precision ckpt_prec; -1 if the run's storage cannot be
- * checkpointed at that precision */
+ * checkpointed at that precision (ckpt_prec is the storage precision, FP16 / BF16 for an FP32 run,
+ * or FP32 for an FP64 run) */
 int64_t fmx_adjoint_ckpt_bytes(const fmx_coupled_desc *desc, int ckpt_prec);
 /* fmx_coupled_run with checkpoints: stores the state before steps step0 + j*k into slot j of ckpt
  * (at ckpt_prec), outlet sums as fmx_coupled_run (row s - step0); result in buffer
@@ -298,9 +309,10 @@ int fmx_adjoint_forward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bu
  * 528 with ad_inlet 2.  It follows the launch shape (FMX_BLOCK / FMX_TILE, as the kernels do).
  * -1 for a bad descriptor. */
 int64_t fmx_adjoint_workspace_bytes(const fmx_coupled_desc *desc);
-/* Segment slots (of fmx_adjoint_state_bytes) a reverse sweep over segments of k steps needs:
- * one level (two_level = 0) or the two-level minimum (two_level = 1); recast = 1 for 16-bit
- * checkpoints of an FP32 run.  -1 for k < 1. */
+/* Segment slots (of fmx_adjoint_state_bytes; of fmx_adjoint_ckpt_bytes(desc, FMX_F32) for an FP64 run
+ * with FP32 checkpoints) a reverse sweep over segments of k steps needs: one level (two_level = 0)
+ * or the two-level minimum (two_level = 1); recast = 1 for 16-bit checkpoints of an FP32 run (0 for
+ * every other pair).  -1 for k < 1. */
 int64_t fmx_adjoint_seg_slots(int32_t k, int recast, int two_level);
 /* Reverse sweep over the steps of fmx_adjoint_forward (same desc, links, field, tau_field,
  * inlet_series and ckpt_prec; f/g/q/phi of bufs are not used).  Segment j is recomputed from ckpt
@@ -310,8 +322,9 @@ int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *b
                          const fmx_adjoint_bufs *adj, void *stream);
 /* Checkpoints of a run (no counterpart in the reference, which keeps no run state): slot `slot` of
  * ckpt (slots of fmx_adjoint_ckpt_bytes(desc, ckpt_prec)) <- the fused state f[cur], g[cur], q,
- * phi[cur], as stored or (FP32 run) as FP16 / BF16 deviations.  fmx_checkpoint_load puts it back
- * (a 16-bit slot decoded to FP32) and clears the frozen-solid link bits, so fmx_coupled_run goes on
+ * phi[cur], as stored, (FP32 run) as FP16 / BF16 deviations or (FP64 run) rounded to FP32.
+ * fmx_checkpoint_load puts it back (a 16-bit slot decoded to FP32, an FP32 slot widened to FP64) and
+ * clears the frozen-solid link bits, so fmx_coupled_run goes on
  * from it as from the step it was saved before (exactly, for a checkpoint at the storage precision). */
 int fmx_checkpoint_save(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *ckpt, int ckpt_prec,
                         int64_t slot, void *stream);

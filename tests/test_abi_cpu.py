@@ -20,7 +20,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(names) == set(abi._SIGS), set(names) ^ set(abi._SIGS)
-    assert L.fmx_abi_version() == 2
+    assert L.fmx_abi_version() == 3 == abi.ABI_VERSION
 
 
 def test_validate_rejects_bad_descriptors():
@@ -258,3 +258,31 @@ int main(void) {
                         "-o", str(exe), "-L", os.path.dirname(abi.LIB_PATH), "-lfmx_hip"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every struct the bindings mirror has the header's size and field offsets: a C program
+    compiled against include/fmx_abi.h prints offsetof / sizeof for each ctypes field."""
+    import subprocess
+
+    structs = {"fmx_grid": abi.Grid, "fmx_coupled_desc": abi.CoupledDesc, "fmx_coupled_bufs": abi.CoupledBufs,
+               "fmx_adjoint_bufs": abi.AdjointBufs}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "fmx_abi.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['    return 0;', '}']
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(abi.ROOT, "include"), str(src),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, fname, v = line.split()
+        got[(cname, fname)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

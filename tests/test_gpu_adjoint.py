@@ -12,6 +12,9 @@ concentration, checked against
     run's states agree with the FP64 adjoint to 1e-7 .. 1e-5),
   * itself under a different checkpoint interval (relative 1e-12),
   * the FP32-checkpoint adjoint, for an FP32 run with 16-bit checkpoints and FP32 recompute,
+  * the FP64 adjoint, for the FP32 reverse sweep of an FP64 run (FP32 checkpoints, FP32 recompute
+    and adjoint steps: the forward curve is the FP64 run's, and the FP32 drift is confined to one
+    segment),
 and used through the torch.autograd.Function wrapper.
 """
 import dataclasses
@@ -20,7 +23,7 @@ import numpy as np
 import pytest
 
 from fluidmaxxer_amd import workloads
-from fluidmaxxer_amd.adjoint import AdjointRun, CoupledBreakthrough
+from fluidmaxxer_amd.adjoint import AdjointRun, CoupledBreakthrough, segment_slots
 from fluidmaxxer_amd.coupled import CoupledConfig
 from fluidmaxxer_amd.geometry import sphere_pack
 from oracle import ref_cpu as R
@@ -277,6 +280,43 @@ def test_fp32_adjoint_matches_fp64(pbb_problem):
         assert e < 2e-5  # observed 8.8e-6 (field), 6.2e-6 (series)
 
 
+def test_fp32_sweep_of_fp64_run(pbb_problem):
+    """An FP64 run with FP32 checkpoints: its forward curve is the FP64 run's to the bit, and its
+    gradients (FP32 recompute from the rounded checkpoints, FP32 adjoint steps) agree with the FP64
+    adjoint far inside the 1e-4 of an FP64 finite difference."""
+    g64 = pbb_problem.grads()
+    pm = Problem(adj_config("pbb"), prec="f64", ckpt="f32", target=pbb_problem.target)
+    assert np.array_equal(pm.c0, pbb_problem.c0)
+    gm = pm.grads()
+    for name in ("iso_a", "iso_b"):
+        say(f"  f64 run, f32 sweep vs f64 d L/d {name}: rel {rel(gm[name], g64[name]):.2e}")
+        assert rel(gm[name], g64[name]) < 7e-7, name  # observed 2.7e-7, 3.2e-7
+    for name, bound in (("field", 6e-6), ("series", 1.5e-6)):  # observed 2.6e-6, 6.4e-7
+        e = rel_l2(gm[name], g64[name])
+        say(f"  f64 run, f32 sweep vs f64 {name}: relative l2 {e:.2e}")
+        assert e < bound, name
+
+
+def test_fp32_checkpoint_of_fp64_run_round_trip(pbb_problem):
+    """fmx_checkpoint_save of an FP64 state at FP32 rounds each value to nearest; fmx_checkpoint_load
+    widens it back exactly."""
+    import torch
+
+    from fluidmaxxer_amd import abi
+    run = AdjointRun(adj_config("pbb"), 50, 50, "f64", ckpt="f32")
+    run.forward()
+    sim = run.sim
+    cur = sim.cur
+    before = [t.clone() for t in (sim.f[cur], sim.g[cur], sim.q, sim.phis[cur])]
+    abi.call("fmx_checkpoint_save", sim._desc_p(), sim._bufs_p(), cur, run.ckpt.data_ptr(), abi.F32, 0, sim._s())
+    for t in (sim.f[cur], sim.g[cur], sim.q, sim.phis[cur]):
+        t.fill_(-1.0)
+    abi.call("fmx_checkpoint_load", sim._desc_p(), sim._bufs_p(), cur, run.ckpt.data_ptr(), abi.F32, 0, sim._s())
+    after = (sim.f[cur], sim.g[cur], sim.q, sim.phis[cur])
+    for name, a, b in zip("fgq" + "p", before, after):
+        assert torch.equal(b, a.to(torch.float32).to(torch.float64)), name
+
+
 @pytest.mark.parametrize("k", [1, 7, NSTEPS])
 def test_checkpoint_interval_invariance(pbb_problem, k):
     g = pbb_problem.grads()
@@ -347,7 +387,8 @@ def c4_config(steps):
     return cfg
 
 
-@pytest.mark.parametrize("prec,ckpt,slots", [("f64", None, 14), ("f64", None, 20), ("f32", "f16", 15)])
+@pytest.mark.parametrize("prec,ckpt,slots", [("f64", None, 14), ("f64", None, 20), ("f32", "f16", 15),
+                                             ("f64", "f32", 14)])
 def test_two_level_segments_bit_identical(pbb_problem, prec, ckpt, slots):
     """A segment store of about 2 sqrt(k) slots (sub-checkpoints within each segment, one more forward
     pass) gives the one-level sweep's gradients to the bit: the recomputed states are the same and
@@ -519,7 +560,8 @@ def test_c4_differentiation_fp64_vs_fd():
 def test_c4_fp32_adjoint_vs_fp64_adjoint():
     """C4-scale FP32 adjoint against the FP64 adjoint, loss against 0.85 x the FP64 curve (residuals
     of 15% of the curve and of one sign, so neither the FP32 forward error nor cancellation between
-    steps dominates the gradient)."""
+    steps dominates the gradient); and the FP64 run with FP32 checkpoints and an FP32 reverse sweep,
+    whose FP32 drift is restarted from the FP64 trajectory at every checkpoint."""
     steps = 12000
     cfg = c4_config(steps)
     p64 = Problem(cfg, prec="f64", nsteps=steps, k=200)
@@ -533,6 +575,16 @@ def test_c4_fp32_adjoint_vs_fp64_adjoint():
     e = np.linalg.norm(g32["field"] - g64["field"]) / np.linalg.norm(g64["field"])
     say(f"  C4 f32 vs f64 field gradient: relative l2 {e:.2e}")
     assert e < 4e-3  # observed 1.92e-3
+    del p32
+    pm = Problem(cfg, prec="f64", nsteps=steps, k=200, target=p64.target, ckpt="f32")
+    assert np.array_equal(pm.c0, p64.c0)
+    gm = pm.grads()
+    for name in ("iso_a", "iso_b"):
+        say(f"  C4 f64 run + f32 sweep vs f64 d L/d {name}: rel {rel(gm[name], g64[name]):.2e}")
+        assert rel(gm[name], g64[name]) < 4e-5  # observed 1.5e-5, 1.7e-5
+    e = rel_l2(gm["field"], g64["field"])
+    say(f"  C4 f64 run + f32 sweep vs f64 field gradient: relative l2 {e:.2e}")
+    assert e < 6e-5  # observed 2.5e-5
 
 
 def rel_l2(a, b):
@@ -620,6 +672,16 @@ def test_c5_half_checkpoints_against_fp64():
     # observed: c_out 3.6e-5, knots 2.7e-4, field 2.8e-4, iso 2.6e-4
     assert errs["c_out"] < 1e-4 and errs["knots"] < 6e-4 and errs["field"] < 6e-4, errs
     assert errs["iso_a"] < 6e-4 and errs["iso_b"] < 6e-4, errs
+    del p16
+    # the FP64 run with FP32 checkpoints every 500 steps and an FP32 reverse sweep
+    pm = Problem(cfg, prec="f64", nsteps=steps, k=500, target=p64.target, ckpt="f32")
+    assert np.array_equal(pm.c0, p64.c0)
+    gm = pm.grads()
+    km = B.T @ gm["series"]
+    em = dict(knots=rel_l2(km, k64), field=rel_l2(gm["field"], g64["field"]), iso_a=rel(gm["iso_a"], g64["iso_a"]),
+              iso_b=rel(gm["iso_b"], g64["iso_b"]))
+    say(f"  C5 f64 run + f32 sweep vs f64 adjoint: {({k: f'{v:.2e}' for k, v in em.items()})}")
+    assert max(em.values()) < 1e-4, em  # observed: knots 6.7e-5, field 5.8e-5, iso 1.0e-5
 
 
 def test_c5_full_size_fp16_checkpoints_against_fp64():
@@ -634,7 +696,9 @@ def test_c5_full_size_fp16_checkpoints_against_fp64():
     the inlet value.  There the FP32 forward run itself is about 0.6% off the FP64 curve and its
     gradients about 4.5% off the FP64 adjoint's, with FP32 and FP16 checkpoints alike, so the FP16
     checkpoints' own effect is measured against the FP32-checkpoint run (the forward curves are the same
-    bits).  Bounds: about 2x the measured errors."""
+    bits).  The FP64 run with FP32 checkpoints every 500 steps and an FP32 reverse sweep restarts the
+    FP32 recompute from the FP64 trajectory at every checkpoint; its gradients are compared with the
+    FP64 adjoint directly.  Bounds: about 2x the measured errors."""
     import torch
 
     steps, k = 20000, 500
@@ -647,7 +711,7 @@ def test_c5_full_size_fp16_checkpoints_against_fp64():
     from fluidmaxxer_amd import abi
 
     def grads(prec, ckpt, target):
-        seg = int(abi.lib().fmx_adjoint_seg_slots(k, int(ckpt is not None and ckpt != prec), 1))
+        seg = segment_slots(k, prec, ckpt, two_level=True)
         p = Problem(cfg, prec=prec, nsteps=steps, k=k, target=target, ckpt=ckpt, seg_slots=seg)
         if target is None:
             p.target = 0.85 * p.c0
@@ -663,16 +727,20 @@ def test_c5_full_size_fp16_checkpoints_against_fp64():
     assert g64["c"][-1] < 1e-10  # the front is still inside the pack
     g16 = grads("f32", "f16", g64["target"])
     g32 = grads("f32", None, g64["target"])
+    gm = grads("f64", "f32", g64["target"])
 
     def errs(a, b):
         return dict(knots=rel_l2(a["knots"], b["knots"]), field=rel_l2(a["field"], b["field"]),
                     c_out=rel_l2(a["c"], b["c"]), iso_a=rel(a["iso_a"], b["iso_a"]), iso_b=rel(a["iso_b"], b["iso_b"]))
 
-    e16_64, e32_64, e16_32 = errs(g16, g64), errs(g32, g64), errs(g16, g32)
+    e16_64, e32_64, e16_32, em_64 = errs(g16, g64), errs(g32, g64), errs(g16, g32), errs(gm, g64)
     for name, e in (("f32 run + f16 ckpt vs f64", e16_64), ("f32 run + f32 ckpt vs f64", e32_64),
-                    ("f16 ckpt vs f32 ckpt", e16_32)):
+                    ("f16 ckpt vs f32 ckpt", e16_32), ("f64 run + f32 sweep vs f64", em_64)):
         say(f"  C5 {name}: {({kk: f'{v:.2e}' for kk, v in e.items()})}")
     assert np.array_equal(g16["c"], g32["c"])  # 16-bit checkpoints do not touch the forward run
+    assert np.array_equal(gm["c"], g64["c"])  # nor do FP32 checkpoints of the FP64 run
+    # observed: knots 7.7e-4, field 1.2e-3, iso 1.1e-3 (the FP32 runs: 4.5e-2)
+    assert max(em_64[n] for n in ("knots", "field", "iso_a", "iso_b")) < 2.5e-3, em_64
     assert e16_64["c_out"] < 1.2e-2
     assert max(e16_64[n] for n in ("knots", "field", "iso_a", "iso_b")) < 9e-2, e16_64
     assert e16_32["knots"] < 1.6e-3 and e16_32["field"] < 9e-3, e16_32
