@@ -129,6 +129,13 @@ def test_adjoint_state_bytes_and_argument_checks():
     d64 = cfg.desc(abi.F64)
     assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F16) == -1
     assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F64) == L.fmx_adjoint_state_bytes(C.byref(d64))
+    # FP32 checkpoints of an FP64 run: FP32 state slots; its FP32 sweep needs the FP32 field copies
+    assert L.fmx_adjoint_ckpt_bytes(C.byref(d64), abi.F32) == L.fmx_adjoint_state_bytes(C.byref(d))
+    assert L.fmx_adjoint_ckpt_bytes(C.byref(d), abi.F64) == -1
+    b.field = 0x1000
+    rc = L.fmx_adjoint_backward(C.byref(d64), C.byref(b), 1, 10, 5, 0x1000, abi.F32, 0x1000, 4, C.byref(a), None)
+    assert rc == abi.ERR_ARG and b"field_rev" in L.fmx_last_error()
+    b.field = None
     rc = L.fmx_adjoint_backward(C.byref(d64), C.byref(b), 1, 10, 5, 0x1000, abi.F16, 0x1000, 5, C.byref(a), None)
     assert rc != 0 and b"checkpoint precision" in L.fmx_last_error()
     # a 16-bit checkpoint store needs a segment store even for k = 1
