@@ -11,11 +11,12 @@
 namespace fmx {
 
 // Byte offsets are 32-bit (a scalar plane base plus a vector offset) unless four population planes
-// exceed 4 GiB (WIDE: 64-bit offsets).  FP16 storage is built for 8 waves per SIMD (64 VGPRs, a few
-// bytes of spill; measured 1-3% faster than 7 waves), FP64 for 4 (128 VGPRs instead of 130, at most
-// 20 bytes of spill; 1% faster than 3); FP32 and BF16 keep the compiler's choice.
+// exceed 4 GiB (WIDE: 64-bit offsets).  FP16 and BF16 storage are built for 8 waves per SIMD (64
+// VGPRs, a few bytes of spill; measured 1-3% (FP16) and 0.6% (BF16) faster than 7 waves), FP64 for 4
+// (128 VGPRs instead of 130, at most 20 bytes of spill; 1% faster than 3); FP32 keeps the
+// compiler's choice.
 template <int P, int OP, bool WIDE>
-__global__ void __launch_bounds__(256, P == FMX_F16 ? 8 : (P == FMX_F64 ? 4 : 1)) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
+__global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P == FMX_F64 ? 4 : 1)) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 acc_row) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
