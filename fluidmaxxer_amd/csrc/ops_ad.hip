@@ -167,13 +167,15 @@ __global__ void __launch_bounds__(256) k_unpack_aos(const T *__restrict__ aos, T
     soa[q * n + c] = aos[i];
 }
 
-__global__ void __launch_bounds__(256) k_fill_zero(std::uint32_t *__restrict__ p, i64 words) {
+// Byte fill as kernels (a memset node in a captured graph is what these avoid): 32-bit words of the
+// repeated byte over the aligned middle, single bytes at the two ends.
+__global__ void __launch_bounds__(256) k_fill_words(std::uint32_t *__restrict__ p, i64 words, std::uint32_t pattern) {
     const i64 stride = i64(gridDim.x) * blockDim.x;
-    for (i64 i = gtid(); i < words; i += stride) p[i] = 0u;
+    for (i64 i = gtid(); i < words; i += stride) p[i] = pattern;
 }
-__global__ void k_fill_zero_tail(unsigned char *__restrict__ p, i64 nbytes) {
+__global__ void k_fill_bytes(unsigned char *__restrict__ p, i64 nbytes, unsigned char value) {
     const i64 i = gtid();
-    if (i < nbytes) p[i] = 0;
+    if (i < nbytes) p[i] = value;
 }
 
 // Streaming probes: 16 bytes per lane, grid-stride, non-temporal.  They measure the HBM rate a
@@ -382,28 +384,43 @@ int fmx_unpack_aos(int prec, const void *aos, void *soa, int Q, int64_t n, void 
     FMX_TRY_END
 }
 
+static int fill_bytes(void *dst, unsigned char value, size_t bytes, void *stream, const char *who) {
+    if (bytes == 0) return FMX_OK;
+    if (!dst) return set_error(FMX_ERR_ARG, "%s: null pointer", who);
+    hipStream_t s = as_stream(stream);
+    unsigned char *p = static_cast<unsigned char *>(dst);
+    const i64 head = i64((4 - reinterpret_cast<std::uintptr_t>(p) % 4) % 4) < i64(bytes)
+                         ? i64((4 - reinterpret_cast<std::uintptr_t>(p) % 4) % 4)
+                         : i64(bytes);
+    if (head) {
+        hipLaunchKernelGGL(k_fill_bytes, dim3(1), dim3(4), 0, s, p, head, value);
+        if (int rc = after_launch("k_fill_bytes")) return rc;
+    }
+    const i64 words = (i64(bytes) - head) / 4;
+    if (words) {
+        const unsigned nb = blocks_for(words) < 65536u ? blocks_for(words) : 65536u;
+        hipLaunchKernelGGL(k_fill_words, dim3(nb), dim3(256), 0, s, reinterpret_cast<std::uint32_t *>(p + head), words,
+                           std::uint32_t(value) * 0x01010101u);
+        if (int rc = after_launch("k_fill_words")) return rc;
+    }
+    const i64 tail = i64(bytes) - head - 4 * words;
+    if (tail) {
+        hipLaunchKernelGGL(k_fill_bytes, dim3(1), dim3(4), 0, s, p + head + 4 * words, tail, value);
+        if (int rc = after_launch("k_fill_bytes")) return rc;
+    }
+    return FMX_OK;
+}
+
 int fmx_fill_zero(void *dst, size_t bytes, void *stream) {
     FMX_TRY_BEGIN
-    if (bytes == 0) return FMX_OK;
-    if (!dst) return set_error(FMX_ERR_ARG, "fmx_fill_zero: null pointer");
-    hipStream_t s = as_stream(stream);
-    const std::uintptr_t addr = reinterpret_cast<std::uintptr_t>(dst);
-    if (addr % 4 == 0 && bytes >= 4) {
-        const i64 words = i64(bytes / 4);
-        const unsigned nb = blocks_for(words) < 65536u ? blocks_for(words) : 65536u;
-        hipLaunchKernelGGL(k_fill_zero, dim3(nb), dim3(256), 0, s, static_cast<std::uint32_t *>(dst), words);
-        int rc = after_launch("k_fill_zero");
-        if (rc) return rc;
-        const i64 tail = i64(bytes % 4);
-        if (tail) {
-            hipLaunchKernelGGL(k_fill_zero_tail, dim3(1), dim3(4), 0, s, static_cast<unsigned char *>(dst) + words * 4, tail);
-            return after_launch("k_fill_zero_tail");
-        }
-        return FMX_OK;
-    }
-    hipLaunchKernelGGL(k_fill_zero_tail, dim3(blocks_for(i64(bytes))), dim3(256), 0, s, static_cast<unsigned char *>(dst),
-                       i64(bytes));
-    return after_launch("k_fill_zero_tail");
+    return fill_bytes(dst, 0, bytes, stream, "fmx_fill_zero");
+    FMX_TRY_END
+}
+
+// cudaMemset's contract (the low byte of value in every byte), done by the fill kernels above
+int fmx_memset(void *dst, int value, size_t bytes, void *stream) {
+    FMX_TRY_BEGIN
+    return fill_bytes(dst, static_cast<unsigned char>(value & 0xff), bytes, stream, "fmx_memset");
     FMX_TRY_END
 }
 

@@ -117,12 +117,6 @@ int fmx_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
     return copy(dst, src, bytes, hipMemcpyDeviceToDevice, stream, "fmx_memcpy_d2d");
 }
 
-int fmx_memset(void *dst, int value, size_t bytes, void *stream) {
-    if (bytes == 0) return FMX_OK;
-    if (!dst) return set_error(FMX_ERR_ARG, "fmx_memset: null pointer");
-    return check_hip(hipMemsetAsync(dst, value, bytes, as_stream(stream)), "hipMemsetAsync");
-}
-
 int fmx_stream_create(void **stream) {
     if (!stream) return set_error(FMX_ERR_ARG, "fmx_stream_create: null pointer");
     hipStream_t s = nullptr;

@@ -74,8 +74,9 @@ int fmx_free(void *ptr);
 int fmx_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int fmx_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int fmx_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* cudaMemset semantics (every byte set to value & 0xff), done by a fill kernel rather than a memset
+ * node, so it is safe inside captured graphs; fmx_fill_zero is fmx_memset(dst, 0, ...) */
 int fmx_memset(void *dst, int value, size_t bytes, void *stream);
-/* zero-fill with a kernel (safe inside captured graphs) */
 int fmx_fill_zero(void *dst, size_t bytes, void *stream);
 int fmx_stream_create(void **stream);
 int fmx_stream_destroy(void *stream);

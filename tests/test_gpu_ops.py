@@ -309,3 +309,52 @@ def test_sync_check_mode():
         results.append((c, host(sim.phi), host(fout)))
     for a, b in zip(*results):
         assert np.array_equal(a, b)
+
+
+def test_memset_and_fill_zero_bytes():
+    """fmx_memset sets every byte to value & 0xff (cudaMemset's contract) and fmx_fill_zero to 0, at
+    every start alignment and length around the 4-byte words the fill kernel writes; the bytes on either
+    side stay untouched."""
+    t = torch()
+    say("fmx_memset / fmx_fill_zero over offsets 0..7 and lengths 0..70, 4099")
+    buf = t.empty(8192, dtype=t.uint8, device="cuda")
+    for off in range(8):
+        for length in list(range(0, 71)) + [4099]:
+            for value, fn in ((0x1A5, "fmx_memset"), (0, "fmx_fill_zero")):
+                buf.fill_(0x33)
+                ptr = buf.data_ptr() + off
+                if fn == "fmx_memset":
+                    abi.call("fmx_memset", ptr, value, length, stream())
+                else:
+                    abi.call("fmx_fill_zero", ptr, length, stream())
+                sync()
+                h = host(buf)
+                want = np.full(8192, 0x33, np.uint8)
+                want[off:off + length] = value & 0xFF
+                assert np.array_equal(h, want), (fn, off, length)
+
+
+def test_memset_in_a_captured_graph():
+    """fmx_memset and fmx_fill_zero captured into a HIP graph (they launch kernels, not memset nodes):
+    three replays, each after eager work that overwrites the buffers, each checked."""
+    t = torch()
+    say("fmx_memset / fmx_fill_zero under torch.cuda.graph capture, 3 replays")
+    a = t.empty(1 << 20, dtype=t.uint8, device="cuda")
+    b = t.empty((1 << 20) + 3, dtype=t.uint8, device="cuda")
+    a.fill_(1)
+    b.fill_(1)
+    sync()
+    g = t.cuda.CUDAGraph()
+    with t.cuda.graph(g):
+        s = t.cuda.current_stream().cuda_stream
+        abi.call("fmx_memset", a.data_ptr() + 1, 0x7E, a.numel() - 2, s)
+        abi.call("fmx_fill_zero", b.data_ptr(), b.numel(), s)
+    for rep in range(3):
+        a.fill_(rep + 2)
+        b.fill_(rep + 2)
+        t.cuda.synchronize()
+        g.replay()
+        t.cuda.synchronize()
+        ha, hb = host(a), host(b)
+        assert ha[0] == rep + 2 and ha[-1] == rep + 2 and np.all(ha[1:-1] == 0x7E), rep
+        assert not hb.any(), rep
