@@ -19,6 +19,7 @@ template <class C> struct CoupledParams {
     int xcd_remap;
     int tile_y, tile_z;      // row tiles (0: plain y-then-z order); they divide gy and gz
     int pair;                // two cells per thread (k_coupled_pair) where the storage and nx allow it
+    int freeze;              // frozen solid cells may be present (FMX_FREEZE=1): the step checks their link bit
     int lane_bc;             // boundary lanes take their x +- 1 neighbour's values by lane shuffles
     i64 n;
     C omega, rho0, uin_x, uin_y, uin_z, uin2, rho_out, phi_in, omega_ad, min_conc, iso_a, iso_b, k_ldf, f_ref;
@@ -214,7 +215,8 @@ template <class C> CoupledParams<C> make_params(const fmx_coupled_desc &d) {
     // runs only in the one-cell-per-thread kernel
     const char *pe = std::getenv("FMX_PAIR");
     const char *fz = std::getenv("FMX_FREEZE");
-    p.pair = (pe && std::atoi(pe) != 0) && !(fz && std::atoi(fz) != 0) && d.nx >= 128 && d.nx % 128 == 0;
+    p.freeze = (fz && std::atoi(fz) != 0) ? 1 : 0;
+    p.pair = (pe && std::atoi(pe) != 0) && !p.freeze && d.nx >= 128 && d.nx % 128 == 0;
     // the x = nx-2 / nx-1 and x = 0 / 1 cells share a wave when rows start at wave boundaries and
     // nx - 1 is not a multiple of 64
     p.lane_bc = (ls.bx % 64 == 0) && ((d.nx - 1) % 64 != 0);

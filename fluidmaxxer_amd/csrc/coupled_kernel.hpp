@@ -11,12 +11,16 @@
 namespace fmx {
 
 // Byte offsets are 32-bit (a scalar plane base plus a vector offset) unless four population planes
-// exceed 4 GiB (WIDE: 64-bit offsets).  FP16 and BF16 storage are built for 8 waves per SIMD (64
-// VGPRs, a few bytes of spill; measured 1-3% (FP16) and 0.6% (BF16) faster than 7 waves), FP64 for 4
-// (128 VGPRs instead of 130, at most 20 bytes of spill; 1% faster than 3); FP32 keeps the
-// compiler's choice.
-template <int P, int OP, bool WIDE>
-__global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P == FMX_F64 ? 4 : 1)) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
+// exceed 4 GiB (WIDE: 64-bit offsets).  FP16 and BF16 storage are built for 7 waves per SIMD (72
+// VGPRs): with every pull and concentration gather issued as one run of loads, 8 waves (64 VGPRs)
+// spill and run 25% slower, 6 waves 1.5% slower (512^3).  FP64 for 4 (128 VGPRs instead of 130, at
+// most 20 bytes of spill; 1% faster than 3); FP32 keeps the compiler's choice.
+// FZ: frozen solid cells may be present (FMX_FREEZE=1); without it the frozen link bit is not
+// looked at, and the pulls are not wrapped in a branch that is decided lane by lane.  (Ignoring the bit
+// is always correct: a frozen cell's two buffers hold its two alternating states, which the full path
+// reproduces.)
+template <int P, int OP, bool WIDE, bool FZ>
+__global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P == FMX_F64 ? 4 : 1)) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
                                                  i64 acc_row) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
@@ -39,7 +43,7 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P 
     const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
     // a frozen solid cell alternates between the two states held in f and f_new and has zero
     // post-collision velocity in both: no NS loads, collision or stores
-    const bool frozen = (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
+    const bool frozen = FZ && (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
 
     // Source of the pull along (cx, cy, cz): the cell - c, or the cell itself when that lies outside
     // the box (x is never periodic; y / z wrap when periodic).  The row part is wave-uniform: a scalar
@@ -59,10 +63,20 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P 
         if (!zin[2]) dz[2] = nz - 1;
         zin[0] = zin[2] = true;
     }
+    // byte delta of row (y - cy, z - cz) for cy, cz in {-1, 0, 1}, computed once (wave-uniform) so the
+    // pulls below form one run of loads with no branches between them
+    Off rowd[3][3];
+    bool rowok[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            rowd[a][b] = (Off(dy[a]) * Off(nx) + Off(dz[b]) * (Off(nx) * Off(ny))) * Off(SB);
+            rowok[a][b] = yin[a] && zin[b];
+        }
     auto src_b = [&](int cx, int cy, int cz) -> Off {  // byte offset of the source cell within a plane
-        const bool ok = yin[cy + 1] && zin[cz + 1];
-        const Off r = Off(dy[cy + 1]) * Off(nx) + Off(dz[cz + 1]) * (Off(nx) * Off(ny));
-        const Off d = ok ? (r - Off(cx)) * Off(SB) : Off(0);
+        const bool ok = rowok[cy + 1][cz + 1];
+        const Off d = ok ? rowd[cy + 1][cz + 1] - Off(cx) * Off(SB) : Off(0);
         if (cx == 0) return cell_b + d;
         return cell_b + (((cx == 1) ? xm_ok : xp_ok) ? d : Off(0));
     };
@@ -88,7 +102,10 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P 
     });
 
     // ------------------------------------------- AD pull and sorption inputs
-    // (issued with the NS pulls; nothing below writes what they read)
+    // (issued with the NS pulls; nothing below writes what they read).  Every lane loads from an
+    // address in the box and the per-lane choices (series substitution, sorbing cell) are selects,
+    // not branches: a lane-dependent branch around a load closes a basic block, and the loads and
+    // their uses inside it then wait for the memory round trip one direction at a time.
     const C phi_pre = (prm.ad_inlet == 2) ? C(p.inlet_series[step]) : C(0);
     C gi[7];
     static_for<0, 7>([&](auto qc) {
@@ -97,27 +114,23 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 8 : (P 
         const Off s = src_b(cx, cy, cz);
         // x of the source (or keep-local) cell: x - cx when in the domain, else x
         const int sx = (cx == 1) ? (x > 0 ? x - 1 : x) : ((cx == -1) ? (x < nx - 1 ? x + 1 : x) : x);
-        if (prm.ad_inlet == 2 && sx == 0) {
-            gi[q] = phi_pre * w7<C>(q);
+        const S raw = ld_b(plane(p.g, q, n), s);
+        C v;
+        if constexpr (IsHalf<P>::value) {
+            // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
+            // populations of its x - 1 neighbour, stored about that neighbour's concentration
+            Off sref = s * (CB / SB);
+            if (prm.ad_outlet && sx == nx - 1) sref -= CB;
+            v = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
         } else {
-            const S raw = ld_b(plane(p.g, q, n), s);
-            if constexpr (IsHalf<P>::value) {
-                // deviation about the source cell's lagged concentration; a Neumann outlet cell holds the
-                // populations of its x - 1 neighbour, stored about that neighbour's concentration
-                Off sref = s * (CB / SB);
-                if (prm.ad_outlet && sx == nx - 1) sref -= CB;
-                gi[q] = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
-            } else {
-                gi[q] = raw;
-            }
+            v = raw;
         }
+        gi[q] = (prm.ad_inlet == 2 && sx == 0) ? phi_pre * w7<C>(q) : v;
     });
     const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
-    C phi_lag = C(0), q_old = C(0);
-    if (sorbs) {
-        phi_lag = ld_b(p.phi, cell * CB);
-        q_old = ld_b(p.q, cell * CB);
-    }
+    // read by every lane (the sorption region is most of a pack's waves); used only where sorbs
+    const C phi_lag = ld_b(p.phi, cell * CB);
+    const C q_old = ld_b(p.q, cell * CB);
 
     // ------------------------------------------ NS collide (in place) and moments
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
@@ -269,19 +282,24 @@ int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParam
     // (FMX_WIDE=1 forces the 64-bit form, to check it against the 32-bit one on small boxes)
     const char *fw = std::getenv("FMX_WIDE");
     const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(typename Storage<P>::S)) > (i64(1) << 32);
-#define FMX_LAUNCH(OP, W) hipLaunchKernelGGL((k_coupled<P, OP, W>), grid, block, 0, s, ptr, prm, step, acc_row)
+#define FMX_LAUNCH(OP, W, FZ) hipLaunchKernelGGL((k_coupled<P, OP, W, FZ>), grid, block, 0, s, ptr, prm, step, acc_row)
     switch (ns_op) {
         case FMX_NS_BGK:
-            if (wide) FMX_LAUNCH(FMX_NS_BGK, true);
-            else FMX_LAUNCH(FMX_NS_BGK, false);
+            if (prm.freeze) {
+                if (wide) FMX_LAUNCH(FMX_NS_BGK, true, true);
+                else FMX_LAUNCH(FMX_NS_BGK, false, true);
+            } else {
+                if (wide) FMX_LAUNCH(FMX_NS_BGK, true, false);
+                else FMX_LAUNCH(FMX_NS_BGK, false, false);
+            }
             break;
         case FMX_NS_POROUS:
-            if (wide) FMX_LAUNCH(FMX_NS_POROUS, true);
-            else FMX_LAUNCH(FMX_NS_POROUS, false);
+            if (wide) FMX_LAUNCH(FMX_NS_POROUS, true, false);
+            else FMX_LAUNCH(FMX_NS_POROUS, false, false);
             break;
         case FMX_NS_PBB:
-            if (wide) FMX_LAUNCH(FMX_NS_PBB, true);
-            else FMX_LAUNCH(FMX_NS_PBB, false);
+            if (wide) FMX_LAUNCH(FMX_NS_PBB, true, false);
+            else FMX_LAUNCH(FMX_NS_PBB, false, false);
             break;
         default: return set_error(FMX_ERR_ARG, "fmx_coupled: unknown ns_op %d", ns_op);
     }
