@@ -258,13 +258,15 @@ def adjoint_c5_accuracy(steps=None, k=500):
     }
 
 
-def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", "f16", "bf16")):
+def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", "f16", "bf16", "f16r", "bf16r")):
     """The workload run for `steps` steps (C3: 512^3, 20000) in each storage precision, against the FP64
     run: per-field errors after the last step (rho, u of the populations as the reference holds them, C,
     q), the outlet breakthrough curve C_out(t) of every step, the breakthrough surface C(x, t) -- the
     plane-averaged concentration of every x-plane every `every` steps, which carries the front before it
     reaches the outlet -- and the FP64 sums of C and q (the diagnostic log) at the same times.  Errors:
-    max |a - b| / max |b| (and the relative l2 norm for the fields)."""
+    max |a - b| / max |b| (and the relative l2 norm for the fields).  A trailing "r" (f16r, bf16r): the
+    16-bit storage with its density reference moved to the mean density of the flow every `every` steps
+    (CoupledSim.recenter); without it the reference stays at rho0."""
     import torch
 
     from fluidmaxxer_amd import workloads
@@ -285,24 +287,28 @@ def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", 
     def l2(a, b):
         return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b).clamp_min(1e-300))
 
-    for prec in precs:
+    for name in precs:
+        prec, recentre = name.rstrip("r"), name.endswith("r")
         t0 = time.perf_counter()
         sim = CoupledSim(cfg, prec=prec)
         curve, surface, mass = [], [], []
         for blk in range(nsteps // every):
             curve.append(sim.run(every, mass_every=every))
+            if recentre:
+                sim.recenter()
             surface.append(sim.phi.view(nz, ny, nx).double().mean(dim=(0, 1)))
             mass.append(sim.mass_log[0].clone())
-            print(f"[precision] {prec} step {(blk + 1) * every}/{nsteps} C_out {float(curve[-1][-1]):.3e} "
+            print(f"[precision] {name} step {(blk + 1) * every}/{nsteps} C_out {float(curve[-1][-1]):.3e} "
                   f"front C(x={nx // 4}) {float(surface[-1][nx // 4]):.3e} ({time.perf_counter() - t0:.0f} s)",
                   flush=True)
         curve, surface, mass = torch.cat(curve), torch.stack(surface), torch.stack(mass)
         rho, ux, uy, uz = sim.moments()
         fields = {"rho": rho.double(), "ux": ux.double(), "uy": uy.double(), "uz": uz.double(),
                   "C": sim.phi.double().clone(), "q": sim.q.double().clone()}
+        f_ref = sim.f_ref
         del rho, ux, uy, uz, sim
         torch.cuda.empty_cache()
-        res = {"seconds": round(time.perf_counter() - t0, 1), "c_out_last": float(curve[-1]),
+        res = {"seconds": round(time.perf_counter() - t0, 1), "f_ref": round(f_ref, 6), "c_out_last": float(curve[-1]),
                "front_c_quarter": float(surface[-1][nx // 4]), "nonfinite": float(mass[:, 3].sum())}
         if ref is None:
             ref = dict(fields=fields, curve=curve, surface=surface, mass=mass)
@@ -313,8 +319,8 @@ def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", 
             res["breakthrough_surface"] = linf(surface, ref["surface"])
             res["sum_C_plus_q"] = linf(mass[:, 0] + mass[:, 1], ref["mass"][:, 0] + ref["mass"][:, 1])
             del fields
-        out[prec] = res
-        print(f"[precision] {prec}: {json.dumps(res)}", flush=True)
+        out[name] = res
+        print(f"[precision] {name}: {json.dumps(res)}", flush=True)
     return out
 
 

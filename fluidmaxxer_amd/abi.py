@@ -22,7 +22,7 @@ ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
 OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
 MASS_COLS = 4
-ABI_VERSION = 3  # FMX_ABI_VERSION of the header these structs mirror
+ABI_VERSION = 4  # FMX_ABI_VERSION of the header these structs mirror
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
@@ -127,6 +127,8 @@ _SIGS = {
     "fmx_coupled_view_f": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_coupled_view_g": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_outlet_mean": ([_vp, _i64, _i64, _vp, _vp], _i),
+    "fmx_coupled_recenter_workspace_bytes": ([_DP], C.c_int64),
+    "fmx_coupled_recenter": ([_DP, _BP, _i, _d, C.POINTER(C.c_double), _vp, _i64, _vp], _i),
     "fmx_adjoint_state_bytes": ([_DP], C.c_int64),
     "fmx_adjoint_ckpt_bytes": ([_DP, _i], C.c_int64),
     "fmx_adjoint_forward": ([_DP, _BP, _i64, C.c_int32, C.c_int32, _i, _vp, _i, _vp], _i),

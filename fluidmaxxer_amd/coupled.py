@@ -131,6 +131,7 @@ class CoupledSim:
         self._acc = None
         self._ring = None
         self._mass_ws = None
+        self._recenter_ws = None
         self.mass_log = None   # (rows, 4) float64 device tensor of the last run's periodic sums, or None
         self.stream = torch.cuda.current_stream(dev)
         abi.call("fmx_coupled_build_links", self._desc_p(), self.flags.data_ptr(), self.links.data_ptr(), self._s())
@@ -273,8 +274,29 @@ class CoupledSim:
         out = torch.zeros(abi.MASS_COLS, dtype=torch.float64, device=self.device)
         ws = self._mass_workspace()
         abi.call("fmx_reduce_mass", self.prec, self.phi.data_ptr(), self.q.data_ptr(), self.f[self.cur].data_ptr(),
-                 self.cfg.n, float(self.cfg.f_ref), out.data_ptr(), ws.data_ptr(), ws.numel(), self._s())
+                 self.cfg.n, float(self._desc.f_ref), out.data_ptr(), ws.data_ptr(), ws.numel(), self._s())
         return out
+
+    @property
+    def f_ref(self) -> float:
+        """Density reference of the stored NS populations (16-bit storage: f = w (f_ref + h))."""
+        return float(self._desc.f_ref)
+
+    def recenter(self, target: float = None) -> float:
+        """Moves the density reference of FP16 / BF16 NS storage to `target`, or by default to the mean
+        density of the non-solid cells, re-encoding the current populations (fmx_coupled_recenter); with
+        FP32 / FP64 storage the state is unchanged.  Returns the new reference.  Called every ~1000 steps
+        while a pressure drop builds up, it keeps the stored deviations centred on the flow."""
+        torch = _torch()
+        nb = int(abi.lib().fmx_coupled_recenter_workspace_bytes(self._desc_p()))
+        if self._recenter_ws is None or self._recenter_ws.numel() < nb:
+            self._recenter_ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        out = abi.C.c_double(0.0)
+        tgt = float("nan") if target is None else float(target)
+        abi.call("fmx_coupled_recenter", self._desc_p(), self._bufs_p(), self.cur, tgt, abi.C.byref(out),
+                 self._recenter_ws.data_ptr(), self._recenter_ws.numel(), self._s())
+        self._desc.f_ref = out.value
+        return out.value
 
     def synchronize(self):
         self.stream.synchronize()

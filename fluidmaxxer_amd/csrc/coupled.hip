@@ -34,6 +34,8 @@
 //     FMX_OUTLET_SLOTS FP64 accumulators per step.
 // When x = nx-2 and nx-1 fall in different waves ((nx - 1) % 64 == 0) the x = 1 and x = nx-2
 // threads write the neighbouring boundary cell's populations instead.
+#include <cmath>
+
 #include "coupled_kernel.hpp"
 #include "diagnostics.hpp"
 
@@ -191,6 +193,15 @@ __global__ void __launch_bounds__(256) k_coupled_view(const typename Storage<P>:
         }
         out[q * n + c] = v;
     }
+}
+
+// Moves the density reference of 16-bit NS storage from ref_old to ref_new: h' = h + (ref_old - ref_new)
+// (delta, exact for references within a factor of two) with one rounding to the storage type, so
+// w (ref_new + h') is w (ref_old + h) up to that rounding.  Grid-stride over the 19 n stored values.
+template <int P>
+__global__ void __launch_bounds__(256) k_recenter(typename Storage<P>::S *f, i64 count, float delta) {
+    using S = typename Storage<P>::S;
+    for (i64 i = gtid(); i < count; i += i64(gridDim.x) * blockDim.x) f[i] = S(widen(f[i]) + delta);
 }
 
 // Encode populations into storage: dst[q*n + c] = src ? src[q*n + c] : w_q * value.
@@ -556,6 +567,60 @@ int fmx_coupled_view_g(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int
         case FMX_F16: return view<FMX_F16, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
         default: return view<FMX_BF16, 7>(*d, b->g[cur], nullptr, b->phi[cur], g_out, 0.0, s);
     }
+    FMX_TRY_END
+}
+
+int64_t fmx_coupled_recenter_workspace_bytes(const fmx_coupled_desc *d) {
+    if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    return int64_t(mass_blocks(n) + 1) * kMassCols * int64_t(sizeof(double));
+}
+
+int fmx_coupled_recenter(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, double target,
+                         double *f_ref_out, void *workspace, int64_t workspace_bytes, void *stream) {
+    FMX_TRY_BEGIN
+    int rc = fmx_coupled_validate(d, b);
+    if (rc) return rc;
+    if ((cur != 0 && cur != 1) || !f_ref_out) return set_error(FMX_ERR_ARG, "fmx_coupled_recenter: bad arguments");
+    hipStream_t s = as_stream(stream);
+    const i64 n = i64(d->nx) * d->ny * d->nz;
+    double ref = target;
+    if (std::isnan(target)) {  // the mean density of the cells without the solid link bit
+        const int64_t need = fmx_coupled_recenter_workspace_bytes(d);
+        if (!workspace || workspace_bytes < need)
+            return set_error(FMX_ERR_ARG, "fmx_coupled_recenter: workspace of %lld bytes given, %lld needed "
+                                          "(fmx_coupled_recenter_workspace_bytes)",
+                             (long long)(workspace ? workspace_bytes : 0), (long long)need);
+        double *partial = static_cast<double *>(workspace);
+        double *out = partial + i64(mass_blocks(n)) * kMassCols;
+        if (int rc2 = launch_fluid_rho_any(d->prec, b->f[cur], b->links, n, d->f_ref, partial, out, s)) return rc2;
+        double host[2] = {0.0, 0.0};
+        if (int rc2 = check_hip(hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, s),
+                                "fmx_coupled_recenter: hipMemcpyAsync"))
+            return rc2;
+        if (int rc2 = check_hip(hipStreamSynchronize(s), "fmx_coupled_recenter: hipStreamSynchronize")) return rc2;
+        ref = host[1] > 0.0 ? host[0] / host[1] : d->f_ref;
+    }
+    if (!std::isfinite(ref)) return set_error(FMX_ERR_ARG, "fmx_coupled_recenter: non-finite reference %g", ref);
+    if (d->prec == FMX_F16 || d->prec == FMX_BF16) {
+        if (!(ref > 0.5 * d->f_ref) || !(ref < 2.0 * d->f_ref))
+            return set_error(FMX_ERR_ARG, "fmx_coupled_recenter: reference %g is not within a factor of two of f_ref %g",
+                             ref, d->f_ref);
+        // the kernels decode with the reference rounded to FP32; so does the shift
+        const float delta = float(d->f_ref) - float(ref);
+        const i64 count = 19 * n;
+        const i64 want = (count + 255) / 256;
+        const unsigned grid = unsigned(want < 65536 ? want : 65536);
+        if (d->prec == FMX_F16)
+            hipLaunchKernelGGL(k_recenter<FMX_F16>, dim3(grid), dim3(256), 0, s, static_cast<_Float16 *>(b->f[cur]),
+                               count, delta);
+        else
+            hipLaunchKernelGGL(k_recenter<FMX_BF16>, dim3(grid), dim3(256), 0, s, static_cast<__bf16 *>(b->f[cur]),
+                               count, delta);
+        if (int rc2 = after_launch("k_recenter")) return rc2;
+    }
+    *f_ref_out = ref;
+    return FMX_OK;
     FMX_TRY_END
 }
 

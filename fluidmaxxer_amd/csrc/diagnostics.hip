@@ -84,6 +84,31 @@ __global__ void __launch_bounds__(256) k_mass_final(const double *__restrict__ p
         for (int k = 0; k < kMassCols; ++k) out[k] = tot[k];
 }
 
+// Density sum over the cells whose link word has no solid bit (out[0]) and their number (out[1]),
+// for the deviation reference of 16-bit NS storage; the same fixed-order two-pass reduction.
+template <int P>
+__global__ void __launch_bounds__(256)
+    k_fluid_rho_partial(const typename Storage<P>::S *__restrict__ f, const std::uint32_t *__restrict__ links, i64 n,
+                        typename Storage<P>::C fref, i64 chunk, double *__restrict__ partial) {
+    using C = typename Storage<P>::C;
+    const i64 begin = i64(blockIdx.x) * chunk;
+    const i64 end = begin + chunk < n ? begin + chunk : n;
+    double s[kMassCols] = {0.0, 0.0, 0.0, 0.0};
+    for (i64 c = begin + threadIdx.x; c < end; c += blockDim.x) {
+        if (links[c] & FMX_LINK_SOLID) continue;
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < 19; ++k) r += double(ld_pop<P>(f, i64(k) * n + c, w19<C>(k), fref));
+        s[0] += r;
+        s[1] += 1.0;
+    }
+    double out[kMassCols];
+    block_sums(s, out);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < kMassCols; ++k) partial[i64(blockIdx.x) * kMassCols + k] = out[k];
+}
+
 int mass_blocks(i64 n) {
     const i64 want = (n + kMassCellsPerBlock - 1) / kMassCellsPerBlock;
     return int(want < 1 ? 1 : (want > kMassMaxBlocks ? kMassMaxBlocks : want));
@@ -116,6 +141,31 @@ int launch_mass_any(int prec, const void *phi, const void *q, const void *f, i64
         case FMX_F16: return launch_mass<FMX_F16>(phi, q, f, n, fref, partial, out, s);
         case FMX_BF16: return launch_mass<FMX_BF16>(phi, q, f, n, fref, partial, out, s);
         default: return set_error(FMX_ERR_ARG, "fmx_reduce_mass: unknown precision %d", prec);
+    }
+}
+
+template <int P>
+int launch_fluid_rho(const void *f, const std::uint32_t *links, i64 n, double fref, double *partial, double *out,
+                     hipStream_t s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const int nblocks = mass_blocks(n);
+    const i64 chunk = (n + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(k_fluid_rho_partial<P>, dim3(nblocks), dim3(256), 0, s, static_cast<const S *>(f), links, n,
+                       C(fref), chunk, partial);
+    if (int rc = after_launch("k_fluid_rho_partial")) return rc;
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+    return after_launch("k_mass_final");
+}
+
+int launch_fluid_rho_any(int prec, const void *f, const std::uint32_t *links, i64 n, double fref, double *partial,
+                         double *out, hipStream_t s) {
+    switch (prec) {
+        case FMX_F32: return launch_fluid_rho<FMX_F32>(f, links, n, fref, partial, out, s);
+        case FMX_F64: return launch_fluid_rho<FMX_F64>(f, links, n, fref, partial, out, s);
+        case FMX_F16: return launch_fluid_rho<FMX_F16>(f, links, n, fref, partial, out, s);
+        case FMX_BF16: return launch_fluid_rho<FMX_BF16>(f, links, n, fref, partial, out, s);
+        default: return set_error(FMX_ERR_ARG, "fluid density: unknown precision %d", prec);
     }
 }
 

@@ -1,6 +1,8 @@
 // diagnostics.hpp -- the FP64 state sums of diagnostics.hip, for the fused run's periodic log.
 #pragma once
 
+#include <cstdint>
+
 #include "fmx_internal.hpp"
 
 namespace fmx {
@@ -16,5 +18,10 @@ int mass_blocks(i64 n);
 // into out[0..3], partials in `partial` (mass_blocks(n) * kMassCols doubles)
 int launch_mass_any(int prec, const void *phi, const void *q, const void *f, i64 n, double fref, double *partial,
                     double *out, hipStream_t s);
+
+// FP64 sum of the stored populations over the cells without the solid link bit (out[0]) and their
+// number (out[1]); partials as above, out holds kMassCols doubles
+int launch_fluid_rho_any(int prec, const void *f, const std::uint32_t *links, i64 n, double fref, double *partial,
+                         double *out, hipStream_t s);
 
 }  // namespace fmx

@@ -30,8 +30,9 @@ extern "C" {
  * (outlet_weight_len), checked before any launch; fmx_reduce_mass with a caller workspace and a
  * non-finite count, and its periodic log inside fmx_coupled_run (mass_log).
  * 3: FP32 checkpoints of an FP64 run, whose reverse sweep runs in FP32 (fmx_adjoint_bufs field_rev,
- * tau_field_rev) */
-#define FMX_ABI_VERSION 3
+ * tau_field_rev)
+ * 4: fmx_coupled_recenter (the density reference of 16-bit NS storage moved to the flow's mean) */
+#define FMX_ABI_VERSION 4
 
 /* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
 enum { FMX_F32 = 0, FMX_F64 = 1, FMX_F16 = 2, FMX_BF16 = 3 };
@@ -245,6 +246,18 @@ int fmx_coupled_view_f(const fmx_coupled_desc *desc, const fmx_coupled_bufs *buf
 /* g[cur] converted to compute precision SoA in g_out (7*N). */
 int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *g_out,
                        void *stream);
+/* No counterpart in the reference (its populations are FP32 or FP64).  Moves the density reference
+ * f_ref of FP16 / BF16 NS storage (f = w * (f_ref + h)) to `target`, or, when target is NaN, to the
+ * FP64 mean density of the cells of f[cur] without the FMX_LINK_SOLID bit (a reduction into
+ * `workspace`, fmx_coupled_recenter_workspace_bytes(desc) bytes, then one synchronisation of the
+ * stream); f[cur] is re-encoded about it (h += f_ref - target, one rounding) and *f_ref_out receives
+ * it.  The caller sets desc->f_ref = *f_ref_out before the next call.  Keeping f_ref at the flow's mean
+ * density keeps the stored deviations small where a pressure drop builds up.  FP32 / FP64 storage:
+ * the state is unchanged and *f_ref_out is still the mean (or target).  The reference must stay within
+ * a factor of two of desc->f_ref (FMX_ERR_ARG otherwise). */
+int64_t fmx_coupled_recenter_workspace_bytes(const fmx_coupled_desc *desc);
+int fmx_coupled_recenter(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, double target,
+                         double *f_ref_out, void *workspace, int64_t workspace_bytes, void *stream);
 /* OutletCSV::log average (OutletCSV.hpp:51-80) for nsteps steps from outlet_acc: mean[k] = sum / count */
 int fmx_outlet_mean(const double *outlet_acc, int64_t nsteps, int64_t count, double *mean, void *stream);
 

@@ -20,7 +20,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(names) == set(abi._SIGS), set(names) ^ set(abi._SIGS)
-    assert L.fmx_abi_version() == 3 == abi.ABI_VERSION
+    assert L.fmx_abi_version() == 4 == abi.ABI_VERSION
 
 
 def test_validate_rejects_bad_descriptors():
@@ -44,6 +44,35 @@ def test_validate_rejects_bad_descriptors():
     assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 1
     with pytest.raises(abi.FmxError):
         abi.call("fmx_coupled_run", C.byref(d), C.byref(b), 1, 1, 0, None)
+
+
+def test_recenter_refuses_a_far_reference_before_any_launch():
+    """fmx_coupled_recenter moves the 16-bit density reference only within a factor of two of the current
+    one (the shift h += f_ref - target is then exact in FP32) and never to a non-finite value: FMX_ERR_ARG
+    before anything is launched (these calls run without a GPU).  A missing output pointer is refused too."""
+    n = 8 * 4 * 4
+    cfg = CoupledConfig(8, 4, 4, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
+    L = abi.lib()
+    b = abi.CoupledBufs()
+    for i in range(2):
+        b.f[i] = b.g[i] = b.phi[i] = 0x1000
+    b.q = b.links = b.inlet_raw = 0x1000
+    out = C.c_double(0.0)
+    for prec in (abi.F16, abi.BF16):
+        d = cfg.desc(prec)
+        assert d.f_ref == 1.0
+        for bad in (2.5, 0.4, float("inf")):
+            assert L.fmx_coupled_recenter(C.byref(d), C.byref(b), 0, bad, C.byref(out), None, 0, None) == abi.ERR_ARG
+        assert b"non-finite" in L.fmx_last_error()
+        assert L.fmx_coupled_recenter(C.byref(d), C.byref(b), 0, 1.05, None, None, 0, None) == abi.ERR_ARG
+        # the mean needs the reduction's workspace
+        assert L.fmx_coupled_recenter(C.byref(d), C.byref(b), 0, float("nan"), C.byref(out), None, 0, None) == abi.ERR_ARG
+        assert b"fmx_coupled_recenter_workspace_bytes" in L.fmx_last_error()
+    d = cfg.desc(abi.F32)
+    # FP32 storage: nothing to re-encode, the target is returned as given
+    assert L.fmx_coupled_recenter(C.byref(d), C.byref(b), 0, 3.0, C.byref(out), None, 0, None) == abi.OK
+    assert out.value == 3.0
+    assert L.fmx_coupled_recenter_workspace_bytes(C.byref(d)) == (L.fmx_reduce_mass_workspace_bytes(n) + 32)
 
 
 def test_short_buffers_are_refused_before_any_launch():

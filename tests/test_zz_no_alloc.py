@@ -20,7 +20,7 @@ def _both(sim, n):
     ws = sim._mass_workspace()
     a = torch.full((abi.MASS_COLS,), -1.0, dtype=torch.float64, device=sim.device)
     b = torch.full((abi.MASS_COLS,), -2.0, dtype=torch.float64, device=sim.device)
-    args = (sim.prec, sim.phi.data_ptr(), sim.q.data_ptr(), sim.f[sim.cur].data_ptr(), n, float(sim.cfg.f_ref))
+    args = (sim.prec, sim.phi.data_ptr(), sim.q.data_ptr(), sim.f[sim.cur].data_ptr(), n, sim.f_ref)
     abi.call("fmx_reduce_mass", *args, a.data_ptr(), ws.data_ptr(), ws.numel(), sim._s())
     abi.call("fmx_reduce_mass_static", *args, b.data_ptr(), sim._s())
     return host(a), host(b)
