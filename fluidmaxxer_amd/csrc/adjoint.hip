@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjointPtrs<P> p, CoupledParams
                 // 16-bit states hold g about the source cell's lagged concentration (the x - 1 cell's
                 // on a Neumann outlet plane), as the forward step stores them
                 const Off sref = s * (CB / SB) - ((prm.ad_outlet && sx == nx - 1) ? Off(CB) : Off(0));
-                gq = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
+                gq = decode_g<P>(raw, w7<C>(q), ld_b(p.phi, sref));
             } else {
                 gq = raw;
             }
@@ -548,8 +548,8 @@ __global__ void __launch_bounds__(1024) k_iso_final(const double *__restrict__ p
 }
 
 // 16-bit checkpoints of an FP32 run: f about f_ref and g about the source cell's concentration
-// (this cell's, or the x - 1 cell's on a Neumann outlet plane, whose populations it holds), the
-// layout the 16-bit forward step itself stores; q and phi are copied in FP32.
+// (this cell's, or the x - 1 cell's on a Neumann outlet plane, whose populations it holds) relative to
+// it (encode_g), the layout the 16-bit forward step itself stores; q and phi are copied in FP32.
 template <int CP>
 __global__ void __launch_bounds__(256) k_ckpt_encode(const float *__restrict__ f, const float *__restrict__ g,
                                                      const float *__restrict__ phi,
@@ -563,8 +563,9 @@ __global__ void __launch_bounds__(256) k_ckpt_encode(const float *__restrict__ f
 #pragma unroll
     for (int q = 0; q < 19; ++q) hf[i64(q) * n + c] = S(f[i64(q) * n + c] * float(Q19::inv_w[q]) - f_ref);
     const float ph = phi[sref];
+    const float inv_s = 1.0f / g_scale(ph);
 #pragma unroll
-    for (int q = 0; q < 7; ++q) hg[i64(q) * n + c] = S(g[i64(q) * n + c] * float(Q7::inv_w[q]) - ph);
+    for (int q = 0; q < 7; ++q) hg[i64(q) * n + c] = encode_g<CP>(g[i64(q) * n + c], float(Q7::inv_w[q]), ph, inv_s);
 }
 
 template <int CP>
@@ -580,7 +581,7 @@ __global__ void __launch_bounds__(256) k_ckpt_decode(const typename Storage<CP>:
     for (int q = 0; q < 19; ++q) f[i64(q) * n + c] = w19<float>(q) * (f_ref + widen(hf[i64(q) * n + c]));
     const float ph = phi[sref];
 #pragma unroll
-    for (int q = 0; q < 7; ++q) g[i64(q) * n + c] = w7<float>(q) * (ph + widen(hg[i64(q) * n + c]));
+    for (int q = 0; q < 7; ++q) g[i64(q) * n + c] = decode_g<CP>(hg[i64(q) * n + c], w7<float>(q), ph);
 }
 
 // Element-wise precision change of one state section (FP32 checkpoints of an FP64 run).

@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_coupled_view(const typename Storage<P>:
             v = ld_pop<P>(pop, i64(q) * n + c, w19<C>(q), ref);
             if (inlet_raw && xx == 0) v = inlet_raw[q * np + c / nx];
         } else if constexpr (IsHalf<P>::value) {
-            v = w7<C>(q) * (phi[sref] + C(pop[i64(q) * n + c]));
+            v = decode_g<P>(pop[i64(q) * n + c], w7<C>(q), phi[sref]);
         } else {
             v = pop[i64(q) * n + c];
         }
@@ -210,7 +210,6 @@ __global__ void __launch_bounds__(256) k_encode_pop(typename Storage<P>::S *dst,
                                                     i64 n, typename Storage<P>::C value, typename Storage<P>::C ref,
                                                     const typename Storage<P>::C *phi, int nx, int ad_outlet) {
     using C = typename Storage<P>::C;
-    using S = typename Storage<P>::S;
     const i64 c = gtid();
     if (c >= n) return;
     const i64 xx = c % nx;
@@ -220,8 +219,10 @@ __global__ void __launch_bounds__(256) k_encode_pop(typename Storage<P>::S *dst,
         const C w = (Q == 19) ? w19<C>(q) : w7<C>(q);
         const int iw = (Q == 19) ? Q19::inv_w[q] : Q7::inv_w[q];
         const C v = src ? src[q * n + c] : w * value;
-        if constexpr (Q == 7 && IsHalf<P>::value) dst[i64(q) * n + c] = S(v * C(iw) - phi[sref]);
-        else st_pop<P>(dst, i64(q) * n + c, v, C(iw), ref);
+        if constexpr (Q == 7 && IsHalf<P>::value)
+            dst[i64(q) * n + c] = encode_g<P>(v, C(iw), phi[sref], C(1) / g_scale(phi[sref]));
+        else
+            st_pop<P>(dst, i64(q) * n + c, v, C(iw), ref);
     }
 }
 

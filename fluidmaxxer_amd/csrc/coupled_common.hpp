@@ -133,6 +133,38 @@ __device__ __forceinline__ typename Storage<P>::S encode(typename Storage<P>::C 
     }
 }
 
+// 16-bit storage of an AD population: its deviation from the reference concentration phi_ref (the
+// source cell's lagged concentration), relative to it, g = w (phi_ref + s h) with s = max(phi_ref, 2^-14),
+// so the 16-bit rounding is relative to the local concentration level down to 2^-14, and below it
+// absolute at s 2^-24 = 3.6e-12 (FP16) instead of 2^-24 = 6e-8: a front's tail near a 1e-8 clamp floor
+// keeps its significant bits.  The floor keeps s positive where phi_ref is 0 or negative, and h within
+// FP16's range (|h| < 65504) wherever |g / w| < 4 -- concentrations of order one, as the lattice units
+// here have them (inlet value 1); a larger h would store as Inf, which the mass diagnostic counts.
+template <class C> __device__ __forceinline__ C g_scale(C phi_ref) {
+    return phi_ref > C(0x1p-14) ? phi_ref : C(0x1p-14);
+}
+template <int P>
+__device__ __forceinline__ typename Storage<P>::C decode_g(typename Storage<P>::S v, typename Storage<P>::C w,
+                                                            typename Storage<P>::C phi_ref) {
+    if constexpr (IsHalf<P>::value) return w * (phi_ref + g_scale(phi_ref) * widen(v));
+    else return v;
+}
+// inv_s: 1 / g_scale(phi_ref), computed once per reference by the caller
+template <int P>
+__device__ __forceinline__ typename Storage<P>::S encode_g(typename Storage<P>::C v, typename Storage<P>::C inv_w,
+                                                            typename Storage<P>::C phi_ref,
+                                                            typename Storage<P>::C inv_s) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    if constexpr (IsHalf<P>::value) {
+        C t = (v * inv_w - phi_ref) * inv_s;
+        asm volatile("" : "+v"(t));  // a separate conversion (see encode)
+        return S(t);
+    } else {
+        return v;
+    }
+}
+
 // feq in the BGK form w rho (1 + cu + 0.5 cu^2 - 1.5 u^2), cu = 3 e.u (BGKCollision.cu:166-173)
 template <int q, class C> __device__ __forceinline__ C feq_bgk(C rho, C ux, C uy, C uz, C u2x15) {
     const C eu = dot3<Q19::cx[q], Q19::cy[q], Q19::cz[q]>(ux, uy, uz);

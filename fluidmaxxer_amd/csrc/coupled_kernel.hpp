@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
             // populations of its x - 1 neighbour, stored about that neighbour's concentration
             Off sref = s * (CB / SB);
             if (prm.ad_outlet && sx == nx - 1) sref -= CB;
-            v = decode<P>(raw, w7<C>(q), ld_b(p.phi, sref));
+            v = decode_g<P>(raw, w7<C>(q), ld_b(p.phi, sref));
         } else {
             v = raw;
         }
@@ -224,6 +224,7 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
     // Neumann outlet cell holds its x - 1 neighbour's populations, about that neighbour's concentration)
     const bool ad_in_cell = prm.ad_inlet == 1 && x == 0;
     const bool ad_out_cell = prm.ad_outlet && x == nx - 1;
+    constexpr bool FAST = IsHalf<P>::value;
     if (prm.lane_bc) {
         C gm[7], phim = C(0);
         if (prm.ad_outlet) {
@@ -231,30 +232,30 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
             for (int q = 0; q < 7; ++q) gm[q] = __shfl_up(go[q], 1, 64);
             phim = __shfl_up(phi, 1, 64);
         }
+        const C ref = ad_out_cell ? phim : phi;
+        const C inv_s = FAST ? recip<FAST>(g_scale(ref)) : C(0);
         static_for<0, 7>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C v = go[q], ref = phi;
-            if (ad_in_cell) {
-                v = prm.phi_in * w7<C>(q);
-            } else if (ad_out_cell) {
-                v = gm[q];
-                ref = phim;
-            }
-            st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(v, C(Q7::inv_w[q]), ref));
+            C v = go[q];
+            if (ad_in_cell) v = prm.phi_in * w7<C>(q);
+            else if (ad_out_cell) v = gm[q];
+            st_plane_nt(plane(p.gn, q, n), cell_b, encode_g<P>(v, C(Q7::inv_w[q]), ref, inv_s));
         });
     } else {
+        const C inv_s = FAST ? recip<FAST>(g_scale(phi)) : C(0);
         if (ad_in_cell) {
 #pragma unroll
             for (int q = 0; q < 7; ++q)
-                st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi));
+                st_plane_nt(plane(p.gn, q, n), cell_b, encode_g<P>(prm.phi_in * w7<C>(q), C(Q7::inv_w[q]), phi, inv_s));
         } else if (!ad_out_cell) {
 #pragma unroll
-            for (int q = 0; q < 7; ++q) st_plane_nt(plane(p.gn, q, n), cell_b, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+            for (int q = 0; q < 7; ++q)
+                st_plane_nt(plane(p.gn, q, n), cell_b, encode_g<P>(go[q], C(Q7::inv_w[q]), phi, inv_s));
         }
         if (prm.ad_outlet && x == nx - 2) {
 #pragma unroll
             for (int q = 0; q < 7; ++q)
-                st_b(plane(p.gn, q, n), cell_b + SB, encode<P>(go[q], C(Q7::inv_w[q]), phi));
+                st_b(plane(p.gn, q, n), cell_b + SB, encode_g<P>(go[q], C(Q7::inv_w[q]), phi, inv_s));
         }
     }
 

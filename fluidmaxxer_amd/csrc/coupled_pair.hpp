@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_coupled_pair(CoupledPtrs<P> p, CoupledP
                 // deviation about the source cell's lagged concentration (x - 1 cell's on a Neumann outlet)
                 const unsigned s = in ? src_row + unsigned(xc - cx) : own_row + unsigned(xc);
                 const unsigned sref = (prm.ad_outlet && sx == nx - 1) ? s - 1u : s;
-                return decode<P>(raw, w7<C>(q), p.phi[sref]);
+                return decode_g<P>(raw, w7<C>(q), p.phi[sref]);
             } else {
                 return C(raw);
             }
@@ -249,11 +249,12 @@ __global__ void __launch_bounds__(256) k_coupled_pair(CoupledPtrs<P> p, CoupledP
     // ------------------------------------------------------------- AD stores
     const bool ain0 = prm.ad_inlet == 1 && x0 == 0;     // ADDirichlet inlet value
     const bool aout1 = prm.ad_outlet && x0 + 2 == nx;   // Neumann outlet: copy of the x0 cell's populations
+    const C is0 = FAST ? recip<FAST>(g_scale(phi0)) : C(0), is1 = FAST ? recip<FAST>(g_scale(phi1)) : C(0);
     static_for<0, 7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const C iw = C(Q7::inv_w[q]);
-        const S e0 = encode<P>(ain0 ? prm.phi_in * w7<C>(q) : go0[q], iw, phi0);
-        const S e1 = aout1 ? encode<P>(go0[q], iw, phi0) : encode<P>(go1[q], iw, phi1);
+        const S e0 = encode_g<P>(ain0 ? prm.phi_in * w7<C>(q) : go0[q], iw, phi0, is0);
+        const S e1 = aout1 ? encode_g<P>(go0[q], iw, phi0, is0) : encode_g<P>(go1[q], iw, phi1, is1);
         st_pair(plane(p.gn, q, n), c0, e0, e1);
     });
 
