@@ -176,8 +176,9 @@ typedef struct fmx_coupled_desc {
     double tau_ad;                  /* AD relaxation time when tau_field is NULL */
     double min_conc;                /* LBMParams::minConcentration */
     double iso_a, iso_b, k_ldf;     /* linear: a = KHenry, k = kA; Langmuir: a = qMax, b = bLangmuir, k = kLDF */
-    double f_ref;                   /* half storage of f: f = w * (f_ref + h); half storage of g holds
-                                       g = w * (phi_src + h), phi_src the source cell's lagged
+    double f_ref;                   /* half storage of f: f = w * (f_ref + h) (fmx_coupled_recenter moves
+                                       it); half storage of g holds g = w * (phi_src + s * h),
+                                       s = max(phi_src, 2^-14), phi_src the source cell's lagged
                                        concentration (the x = nx-2 value on a Neumann outlet plane) */
 } fmx_coupled_desc;
 
