@@ -374,6 +374,31 @@ def traffic_from_profile(workload, prec):
     return d.get("hbm_bytes_per_launch"), rel
 
 
+def targets(line):
+    """The 512^3 targets beside what this run measured; the storage error and the gradient accuracy are
+    not measured by the timing run and name the committed record they come from."""
+    out = {"fp32_roofline_frac": {"target": 0.70, "measured": line["roofline"]["frac"]}}
+    if "storage" in line:
+        out["fp16_over_fp32_mlups"] = {"target": 1.6, "measured": line["storage"]["f16_over_f32"]}
+    rel = os.path.join("profiles", "precision_c3.json")
+    if os.path.exists(os.path.join(ROOT, rel)):
+        with open(os.path.join(ROOT, rel)) as fh:
+            study = json.load(fh)
+        for name, label in (("f16r", "recentred density reference"), ("f16", "density reference held at rho0")):
+            if name in study:
+                out[f"fp16_breakthrough_error_vs_fp64 ({label})"] = {
+                    "target": 1e-3, "measured": study[name]["breakthrough_surface"],
+                    "source": f"{rel} (bench.py --precision-study: C(x, t) over 20,000 steps); not this run"}
+    if "adjoint" in line:
+        out["gradient_cost_over_forward"] = {"target": 3.0, "measured": line["adjoint"]["ratio"],
+                                             "run": "FP64 run, FP32 checkpoints and FP32 reverse sweep (C4)"}
+    out["gradient_error_vs_fd"] = {"target": 1e-4,
+                                   "measured": "FP64 adjoint 1e-10 .. 6e-8 against central differences; the FP64 "
+                                               "run with the FP32 sweep 1.5e-5 .. 2.5e-5 against the FP64 adjoint",
+                                   "source": "tests/test_gpu_adjoint.py; not this run"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -542,6 +567,8 @@ def main():
         # adjoint; ratio over the FP64 forward run
         line["adjoint"] = adjoint_timing(prec="f64", ckpt="f32")
         line["adjoint_single_precision"] = {p: adjoint_timing(prec=p) for p in ("f32", "f64")}
+    if args.workload == "C3" and dims is None and args.prec == "f32":
+        line["targets"] = targets(line)
     if not args.no_cpu_baseline:
         line["cpu_baseline"], all_cores = cpu_baseline(cfg, args.prec, args.cpu_seconds)
         if all_cores is not None:
