@@ -743,5 +743,6 @@ def test_c5_full_size_fp16_checkpoints_against_fp64():
     assert max(em_64[n] for n in ("knots", "field", "iso_a", "iso_b")) < 2.5e-3, em_64
     assert e16_64["c_out"] < 1.2e-2
     assert max(e16_64[n] for n in ("knots", "field", "iso_a", "iso_b")) < 9e-2, e16_64
-    assert e16_32["knots"] < 1.6e-3 and e16_32["field"] < 9e-3, e16_32
-    assert e16_32["iso_a"] < 8.4e-4 and e16_32["iso_b"] < 8.4e-4, e16_32
+    # observed (AD checkpoint values relative to their concentration): knots 4.8e-4, field 1.6e-3, iso 9.9e-5
+    assert e16_32["knots"] < 1e-3 and e16_32["field"] < 3.5e-3, e16_32
+    assert e16_32["iso_a"] < 2.5e-4 and e16_32["iso_b"] < 2.5e-4, e16_32
