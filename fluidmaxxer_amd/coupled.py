@@ -284,7 +284,7 @@ class CoupledSim:
 
     def recenter(self, target: float = None) -> float:
         """Moves the density reference of FP16 / BF16 NS storage to `target`, or by default to the mean
-        density of the non-solid cells, re-encoding the current populations (fmx_coupled_recenter); with
+        density of the non-solid cells, re-encoding the stored populations (fmx_coupled_recenter); with
         FP32 / FP64 storage the state is unchanged.  Returns the new reference.  Called every ~1000 steps
         while a pressure drop builds up, it keeps the stored deviations centred on the flow."""
         torch = _torch()

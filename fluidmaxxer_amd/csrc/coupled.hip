@@ -612,13 +612,17 @@ int fmx_coupled_recenter(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, i
         const i64 count = 19 * n;
         const i64 want = (count + 255) / 256;
         const unsigned grid = unsigned(want < 65536 ? want : 65536);
-        if (d->prec == FMX_F16)
-            hipLaunchKernelGGL(k_recenter<FMX_F16>, dim3(grid), dim3(256), 0, s, static_cast<_Float16 *>(b->f[cur]),
-                               count, delta);
-        else
-            hipLaunchKernelGGL(k_recenter<FMX_BF16>, dim3(grid), dim3(256), 0, s, static_cast<__bf16 *>(b->f[cur]),
-                               count, delta);
-        if (int rc2 = after_launch("k_recenter")) return rc2;
+        // both buffers: the other one holds the second phase of any frozen solid cell (FMX_FREEZE), which
+        // no step rewrites; the rest of it is overwritten by the next step anyway
+        for (int k = 0; k < 2; ++k) {
+            if (d->prec == FMX_F16)
+                hipLaunchKernelGGL(k_recenter<FMX_F16>, dim3(grid), dim3(256), 0, s, static_cast<_Float16 *>(b->f[k]),
+                                   count, delta);
+            else
+                hipLaunchKernelGGL(k_recenter<FMX_BF16>, dim3(grid), dim3(256), 0, s, static_cast<__bf16 *>(b->f[k]),
+                                   count, delta);
+            if (int rc2 = after_launch("k_recenter")) return rc2;
+        }
     }
     *f_ref_out = ref;
     return FMX_OK;

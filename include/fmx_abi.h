@@ -251,8 +251,9 @@ int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *buf
  * f_ref of FP16 / BF16 NS storage (f = w * (f_ref + h)) to `target`, or, when target is NaN, to the
  * FP64 mean density of the cells of f[cur] without the FMX_LINK_SOLID bit (a reduction into
  * `workspace`, fmx_coupled_recenter_workspace_bytes(desc) bytes, then one synchronisation of the
- * stream); f[cur] is re-encoded about it (h += f_ref - target, one rounding) and *f_ref_out receives
- * it.  The caller sets desc->f_ref = *f_ref_out before the next call.  Keeping f_ref at the flow's mean
+ * stream); f[0] and f[1] are re-encoded about it (h += f_ref - target, one rounding) and *f_ref_out
+ * receives it.  The caller sets desc->f_ref = *f_ref_out before the next call; a 16-bit checkpoint
+ * (fmx_checkpoint_save) keeps the f_ref it was saved with.  Keeping f_ref at the flow's mean
  * density keeps the stored deviations small where a pressure drop builds up.  FP32 / FP64 storage:
  * the state is unchanged and *f_ref_out is still the mean (or target).  The reference must stay within
  * a factor of two of desc->f_ref (FMX_ERR_ARG otherwise). */
