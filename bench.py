@@ -562,7 +562,7 @@ def main():
         line["storage"] = st
     if not args.no_c2 and args.workload != "C2" and dims is None:
         c2 = workloads.build("C2", steps=max(steps, 100) + 21)
-        line["c2"] = storage_timing(c2, ("f32",), steps=max(steps, 100), warmup=20)
+        line["c2"] = storage_timing(c2, ("f32", "f16"), steps=max(steps, 100), warmup=20)
         del c2
     if not args.no_adjoint and args.prec in ("f32", "f64"):
         # the FP64 run with an FP32 reverse sweep: FP64 forward curve, gradients within 1e-5 of the FP64
