@@ -571,6 +571,14 @@ def main():
         line["adjoint_single_precision"] = {p: adjoint_timing(prec=p) for p in ("f32", "f64")}
     if args.workload == "C3" and dims is None and args.prec == "f32":
         line["targets"] = targets(line)
+        # the practical ceiling of this box, measured in the same process: boxes differ in what a plain
+        # stream reaches, and the step's roofline fraction moves with it
+        probe = stream_probe(gib=2, reps=5)["stream_probe"]
+        line["same_box_streams"] = {
+            "copy_GBps": probe["stream_copy_GBps"], "read_GBps": probe["stream_read_GBps"],
+            "write_GBps": probe["write_GBps"],
+            "step_achieved_over_copy": round(achieved / probe["stream_copy_GBps"], 3),
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"], all_cores = cpu_baseline(cfg, args.prec, args.cpu_seconds)
         if all_cores is not None:
