@@ -4,7 +4,9 @@
 
 Default workload: C3, the 512^3 overlapping sphere pack (porosity 0.4, 134M
 cells) with the Langmuir LDF, FP32 storage -- the largest single-GPU
-configuration.  Prints ONE JSON line: MLUPS over all cells (solids included),
+configuration -- with the library's default NS layout for it (row-compacted NS
+planes that leave the frozen solid cells out; --layout box for the box layout).
+Prints ONE JSON line: MLUPS over all cells (solids included),
 the HBM roofline of the fused kernel (algorithmic bytes per cell update x cells
 / launch time measured with HIP events on the launch stream, against 8 TB/s,
 plus the HBM bytes per launch from the committed rocprofv3 counter summary of
@@ -36,6 +38,18 @@ HBM_PEAK_GBS = 8000.0
 # adsorbed load and lagged concentration read and written once, one flag byte
 B_ALG = {"f32": 225, "f64": 449, "f16": 121, "bf16": 121}
 B_FIELD = {"f32": 4, "f64": 8, "f16": 4, "bf16": 4}  # per-cell porosity / solid-fraction read
+B_NS = {"f32": 152, "f64": 304, "f16": 76, "bf16": 76}  # of B_ALG: the 19 NS populations read and written
+
+
+def compact_bytes(sim, prec):
+    """Algorithmic bytes per cell update of a run in row-compacted NS planes: the NS populations of the
+    active cells only (the frozen solid cells' are never moved), the rest as B_ALG; and the active
+    fraction."""
+    from fluidmaxxer_amd import abi
+    n = sim.cfg.n
+    active = int((sim.links & abi.LINK_FROZEN).eq(0).sum().item())
+    a = active / n
+    return B_ALG[prec] - B_NS[prec] * (1.0 - a), a
 
 
 def cpu_info():
@@ -343,29 +357,34 @@ def time_steps(sim, steps, warmup):
 
 def storage_timing(cfg, precs, steps=100, warmup=20):
     """The same workload for each storage precision: step time, MLUPS and the fraction of the
-    algorithmic roofline (121 B/cell for FP16 / BF16 storage, 449 for FP64)."""
+    algorithmic roofline (121 B/cell for FP16 / BF16 storage, 449 for FP64).  A name with ":box"
+    (e.g. "f32:box") times that storage in the box layout instead of the library's default."""
     import torch
 
     from fluidmaxxer_amd.coupled import CoupledSim
 
     out = {"workload": f"{cfg.meta['workload']}: {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.ns_op} {cfg.isotherm}, "
                        f"porosity {cfg.meta['porosity']:.4f}, {steps} steps after {warmup}"}
-    for prec in precs:
-        sim = CoupledSim(cfg, prec=prec)
+    for name in precs:
+        prec, _, layout = name.partition(":")
+        sim = CoupledSim(cfg, prec=prec, compact=False if layout == "box" else None)
         ms_step, acc = time_steps(sim, steps, warmup)
         b_alg = B_ALG[prec] + (B_FIELD[prec] if cfg.field is not None else 0)
         achieved = b_alg * cfg.n / (ms_step / 1e3) / 1e9
-        out[prec] = {"mlups": round(cfg.n / (ms_step / 1e3) / 1e6, 1), "ms_per_step": round(ms_step, 4),
-                     "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
+        out[name.replace(":", "_")] = {"mlups": round(cfg.n / (ms_step / 1e3) / 1e6, 1),
+                                       "ms_per_step": round(ms_step, 4),
+                                       "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
+                                       "ns_layout": "row-compacted" if sim.compact else "box"}
         del sim, acc
         torch.cuda.empty_cache()
     return out
 
 
-def traffic_from_profile(workload, prec):
+def traffic_from_profile(workload, prec, compact=False):
     """HBM bytes per launch of the fused kernel from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE runs, scaled by a plain copy's reading) of this workload and storage, and its path."""
-    rel = os.path.join("profiles", f"traffic_{workload.lower()}_{prec}.json")
+    WRITE_SIZE runs, scaled by a plain copy's reading) of this workload, storage and NS layout, and its
+    path."""
+    rel = os.path.join("profiles", f"traffic_{'rc_' if compact else ''}{workload.lower()}_{prec}.json")
     path = os.path.join(ROOT, rel)
     if not os.path.exists(path):
         return None, None
@@ -378,8 +397,14 @@ def targets(line):
     """The 512^3 targets beside what this run measured; the storage error and the gradient accuracy are
     not measured by the timing run and name the committed record they come from."""
     out = {"fp32_roofline_frac": {"target": 0.70, "measured": line["roofline"]["frac"]}}
+    if "compact_layout" in line["roofline"]:
+        out["fp32_roofline_frac"]["measured_on_the_bytes_the_compact_layout_moves"] = \
+            line["roofline"]["compact_layout"]["frac"]
     if "storage" in line:
         out["fp16_over_fp32_mlups"] = {"target": 1.6, "measured": line["storage"]["f16_over_f32"]}
+        if "f16_over_f32_box" in line["storage"]:
+            out["fp16_over_fp32_mlups"]["measured_against_fp32_in_the_box_layout"] = \
+                line["storage"]["f16_over_f32_box"]
     rel = os.path.join("profiles", "precision_c3.json")
     if os.path.exists(os.path.join(ROOT, rel)):
         with open(os.path.join(ROOT, rel)) as fh:
@@ -407,6 +432,9 @@ def main():
     ap.add_argument("--workload", default="C3")
     ap.add_argument("--dims", default=None, help="override box, e.g. 256x256x256")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64", "f16", "bf16"])
+    ap.add_argument("--layout", default="default", choices=["default", "box", "compact"],
+                    help="NS layout of the timed sim: the library's default for the storage (row-compacted "
+                         "for FP32 / FP64, box for FP16 / BF16), or forced")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="CPU baseline budget (one serial step is always timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -487,7 +515,7 @@ def main():
     steps = args.steps if args.steps is not None else min(spec_steps, 200)
     cfg = workloads.build(args.workload, dims=dims, steps=max(steps + args.warmup + 1, spec_steps))
     n = cfg.n
-    sim = CoupledSim(cfg, prec=args.prec)
+    sim = CoupledSim(cfg, prec=args.prec, compact={"default": None, "box": False, "compact": True}[args.layout])
     log = not args.no_outlet
 
     # warm-up (untimed), then exactly `steps` steps between two events on the launch stream
@@ -514,7 +542,9 @@ def main():
         torch.cuda.synchronize()
         c_out_last = float(mean[-1].item())
 
-    traffic, traffic_src = traffic_from_profile(args.workload, args.prec) if dims is None else (None, None)
+    compact = sim.compact
+    b_moved, active = compact_bytes(sim, args.prec) if compact else (b_alg, 1.0)
+    traffic, traffic_src = traffic_from_profile(args.workload, args.prec, compact) if dims is None else (None, None)
     line = {
         "metric": "Coupled D3Q19+D3Q7 MLUPS & %HBM roofline; adjoint/forward time ratio",
         "value": round(mlups, 1),
@@ -550,19 +580,34 @@ def main():
             "traffic_source": (f"{traffic_src}: rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this kernel and workload, "
                                "scaled by a plain copy's reading; not measured in this run") if traffic_src else None,
             "bytes_per_cell_update": b_alg,
-            "kernel": "k_coupled (one launch per step)",
+            "kernel": ("k_coupled_rc (row-compacted NS planes; one launch per step, plus one row-table build and "
+                       "one copy in and out per run)") if compact else "k_coupled (one launch per step)",
         },
     }
+    if compact:
+        # the same step time over the bytes the row-compacted layout moves: the frozen solid cells' NS
+        # populations (B_NS of each) are neither read nor written
+        line["roofline"]["compact_layout"] = {
+            "active_fraction": round(active, 4),
+            "bytes_per_cell_update": round(b_moved, 1),
+            "achieved": round(b_moved * n / (ms_step / 1e3) / 1e9, 1),
+            "frac": round(b_moved * n / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        }
+    line["config"]["ns_layout"] = "row-compacted (frozen solid cells left out)" if compact else "box"
     del sim, acc
     torch.cuda.empty_cache()
     if not args.no_storage_sweep and args.prec == "f32":
-        st = storage_timing(cfg, ("f16", "bf16"), steps=steps, warmup=min(args.warmup, 20))
+        extra = ("f16", "bf16", "f32:box") if compact else ("f16", "bf16")
+        st = storage_timing(cfg, extra, steps=steps, warmup=min(args.warmup, 20))
         st["f16_over_f32"] = round(st["f16"]["mlups"] / mlups, 3)
         st["bf16_over_f32"] = round(st["bf16"]["mlups"] / mlups, 3)
+        if compact:
+            st["f16_over_f32_box"] = round(st["f16"]["mlups"] / st["f32_box"]["mlups"], 3)
+            st["f32_over_f32_box"] = round(mlups / st["f32_box"]["mlups"], 3)
         line["storage"] = st
     if not args.no_c2 and args.workload != "C2" and dims is None:
         c2 = workloads.build("C2", steps=max(steps, 100) + 21)
-        line["c2"] = storage_timing(c2, ("f32", "f16"), steps=max(steps, 100), warmup=20)
+        line["c2"] = storage_timing(c2, ("f32", "f32:box", "f16"), steps=max(steps, 100), warmup=20)
         del c2
     if not args.no_adjoint and args.prec in ("f32", "f64"):
         # the FP64 run with an FP32 reverse sweep: FP64 forward curve, gradients within 1e-5 of the FP64
@@ -579,6 +624,9 @@ def main():
             "write_GBps": probe["write_GBps"],
             "step_achieved_over_copy": round(achieved / probe["stream_copy_GBps"], 3),
         }
+        if compact:
+            line["same_box_streams"]["step_moved_over_copy"] = round(
+                line["roofline"]["compact_layout"]["achieved"] / probe["stream_copy_GBps"], 3)
     if not args.no_cpu_baseline:
         line["cpu_baseline"], all_cores = cpu_baseline(cfg, args.prec, args.cpu_seconds)
         if all_cores is not None:

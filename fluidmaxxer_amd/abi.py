@@ -20,9 +20,10 @@ F32, F64, F16, BF16 = 0, 1, 2, 3
 NS_BGK, NS_POROUS, NS_PBB = 0, 1, 2
 ISO_NONE, ISO_LINEAR, ISO_LANGMUIR = 0, 1, 2
 FLAG_SOLID, FLAG_REGION = 1, 2
+LINK_FROZEN = 1 << 21  # FMX_LINK_FROZEN
 OUTLET_SLOTS, OUTLET_STRIDE, OUTLET_RING_ROWS = 64, 16, 64
 MASS_COLS = 4
-ABI_VERSION = 4  # FMX_ABI_VERSION of the header these structs mirror
+ABI_VERSION = 5  # FMX_ABI_VERSION of the header these structs mirror
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 
 PREC_NAMES = {"f32": F32, "f64": F64, "f16": F16, "bf16": BF16}
@@ -50,6 +51,7 @@ class CoupledDesc(C.Structure):
         ("rho_out", C.c_double), ("phi_in", C.c_double), ("tau_ad", C.c_double), ("min_conc", C.c_double),
         ("iso_a", C.c_double), ("iso_b", C.c_double), ("k_ldf", C.c_double),
         ("f_ref", C.c_double),
+        ("freeze_solids", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 
@@ -62,6 +64,7 @@ class CoupledBufs(C.Structure):
         ("outlet_ring", C.c_void_p), ("inlet_series_len", C.c_int64), ("outlet_acc_rows", C.c_int64),
         ("mass_log", C.c_void_p), ("mass_workspace", C.c_void_p), ("mass_log_rows", C.c_int64),
         ("mass_workspace_bytes", C.c_int64), ("mass_every", C.c_int64),
+        ("compact_work", C.c_void_p), ("compact_work_bytes", C.c_int64),
     ]
 
 
@@ -128,6 +131,7 @@ _SIGS = {
     "fmx_coupled_view_g": ([_DP, _BP, _i, _vp, _vp], _i),
     "fmx_outlet_mean": ([_vp, _i64, _i64, _vp, _vp], _i),
     "fmx_coupled_recenter_workspace_bytes": ([_DP], C.c_int64),
+    "fmx_coupled_compact_bytes": ([_DP, _i64], C.c_int64),
     "fmx_coupled_recenter": ([_DP, _BP, _i, _d, C.POINTER(C.c_double), _vp, _i64, _vp], _i),
     "fmx_adjoint_state_bytes": ([_DP], C.c_int64),
     "fmx_adjoint_ckpt_bytes": ([_DP, _i], C.c_int64),

@@ -60,7 +60,7 @@ class AdjointRun:
         if cfg.ad_inlet == "series" and (cfg.inlet_series is None or len(cfg.inlet_series) < nsteps + 1):
             raise abi.FmxError("inlet_series must hold steps 0..nsteps")
         self.cfg, self.nsteps, self.k = cfg, int(nsteps), int(k)
-        self.sim = CoupledSim(cfg, prec=prec, device=device)
+        self.sim = CoupledSim(cfg, prec=prec, device=device, compact=False)  # every cell on the recorded path
         self.device = self.sim.device
         self.cdt = self.sim.cdt
         # an FP64 run with FP32 checkpoints sweeps back in FP32: its adjoint state and segment slots

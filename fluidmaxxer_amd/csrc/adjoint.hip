@@ -828,11 +828,13 @@ static int forward_ckpt(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i6
                         void *ckpt, int cp, hipStream_t s) {
     const i64 n = i64(d.nx) * d.ny * d.nz;
     if (int rc = clear_frozen(d, b.links, s)) return rc;
+    fmx_coupled_bufs bf = b;
+    bf.compact_work = nullptr;  // every cell is on the full path while it is recorded for the sweep
     const int nseg = (nsteps + k - 1) / k;
     for (int j = 0; j < nseg; ++j) {
         const int len = (j + 1) * k <= nsteps ? k : nsteps - j * k;
         if (int rc = save_ckpt<P>(d, b, cur, ckpt, cp, j, n, s)) return rc;
-        if (int rc = run_coupled_steps<P>(d, b, step0 + i64(j) * k, len, cur, s, i64(j) * k)) return rc;
+        if (int rc = run_coupled_steps<P>(d, bf, step0 + i64(j) * k, len, cur, s, i64(j) * k)) return rc;
         cur = (cur + len) & 1;
     }
     return FMX_OK;
@@ -1073,23 +1075,27 @@ int fmx_checkpoint_load(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, in
     const i64 n = i64(d->nx) * d->ny * d->nz;
     hipStream_t s = as_stream(stream);
     void *ck = const_cast<void *>(ckpt);
-    // frozen solid cells keep their other phase in f[1 - cur], which a checkpoint does not hold: the
-    // run goes on with every cell on the full update (the same values)
-    if (int rc = clear_frozen(*d, b->links, s)) return rc;
+    int rc = FMX_OK;
     if (d->prec == FMX_F32 && ckpt_prec != FMX_F32) {
         const StateView<FMX_F32> dst{static_cast<float *>(b->f[cur]), static_cast<float *>(b->g[cur]),
                                      static_cast<float *>(b->q), static_cast<float *>(b->phi[cur])};
-        if (ckpt_prec == FMX_F16) return load_half_ckpt<FMX_F16>(*d, state_at<FMX_F16>(ck, slot, n), dst, n, s);
-        return load_half_ckpt<FMX_BF16>(*d, state_at<FMX_BF16>(ck, slot, n), dst, n, s);
+        rc = ckpt_prec == FMX_F16 ? load_half_ckpt<FMX_F16>(*d, state_at<FMX_F16>(ck, slot, n), dst, n, s)
+                                  : load_half_ckpt<FMX_BF16>(*d, state_at<FMX_BF16>(ck, slot, n), dst, n, s);
+    } else if (d->prec == FMX_F64 && ckpt_prec == FMX_F32) {
+        rc = load_narrow_ckpt(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
+    } else {
+        switch (d->prec) {
+            case FMX_F32: rc = restore_state<FMX_F32>(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s); break;
+            case FMX_F64: rc = restore_state<FMX_F64>(*b, cur, state_at<FMX_F64>(ck, slot, n), n, s); break;
+            case FMX_F16: rc = restore_state<FMX_F16>(*b, cur, state_at<FMX_F16>(ck, slot, n), n, s); break;
+            default: rc = restore_state<FMX_BF16>(*b, cur, state_at<FMX_BF16>(ck, slot, n), n, s);
+        }
     }
-    if (d->prec == FMX_F64 && ckpt_prec == FMX_F32)
-        return load_narrow_ckpt(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
-    switch (d->prec) {
-        case FMX_F32: return restore_state<FMX_F32>(*b, cur, state_at<FMX_F32>(ck, slot, n), n, s);
-        case FMX_F64: return restore_state<FMX_F64>(*b, cur, state_at<FMX_F64>(ck, slot, n), n, s);
-        case FMX_F16: return restore_state<FMX_F16>(*b, cur, state_at<FMX_F16>(ck, slot, n), n, s);
-        default: return restore_state<FMX_BF16>(*b, cur, state_at<FMX_BF16>(ck, slot, n), n, s);
-    }
+    if (rc) return rc;
+    // frozen solid cells keep their other phase in f[1 - cur], which a checkpoint does not hold: they
+    // are marked again from the restored state (desc.freeze_solids), or every cell goes on with the
+    // full update (the same values)
+    return mark_frozen(*d, *b, cur, s);
     FMX_TRY_END
 }
 

@@ -293,21 +293,217 @@ extern template int launch_coupled_step<FMX_F16>(int, const CoupledPtrs<FMX_F16>
 extern template int launch_coupled_step<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64,
                                                   i64, hipStream_t);
 
+template int launch_coupled_rc<FMX_F32>(const CoupledPtrs<FMX_F32> &, const CoupledParams<float> &, i64, i64,
+                                        const RcArgs &, hipStream_t);
+template int launch_coupled_rc<FMX_F64>(const CoupledPtrs<FMX_F64> &, const CoupledParams<double> &, i64, i64,
+                                        const RcArgs &, hipStream_t);
+extern template int launch_coupled_rc<FMX_F16>(const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
+                                               const RcArgs &, hipStream_t);
+extern template int launch_coupled_rc<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
+                                                const RcArgs &, hipStream_t);
+
+// ------------------------------------------------------------ row-compacted NS storage
+// Workspace (fmx_coupled_bufs::compact_work): the RowSeg table, the scan's chunk sums, a header whose
+// first word is the number of active cells, then the two compact population buffers of 19 planes
+// each, whose plane stride nc is what the remaining bytes hold (a multiple of 128 cells).
+constexpr int kRcChunk = 4096;  // table entries per block of the scan (256 threads x 16)
+// Shorter runs stay in the box layout: building the table, the host synchronisation and the two
+// copies cost about two compact steps per run.
+constexpr int kRcMinSteps = 16;
+
+struct RcLayout {
+    i64 ents, chunks, part_off, hdr_off, planes_off, nc;
+};
+static i64 align256(i64 v) { return (v + 255) & ~i64(255); }
+static RcLayout rc_layout(const fmx_coupled_desc &d, i64 bytes) {
+    RcLayout l{};
+    const i64 nseg = (d.nx + 63) / 64;
+    l.ents = i64(d.ny) * d.nz * nseg;
+    l.chunks = (l.ents + kRcChunk - 1) / kRcChunk;
+    l.part_off = align256(l.ents * i64(sizeof(RowSeg)));
+    l.hdr_off = l.part_off + align256(l.chunks * 4);
+    l.planes_off = l.hdr_off + 256;
+    const i64 sb = d.prec == FMX_F64 ? 8 : (d.prec == FMX_F32 ? 4 : 2);
+    const i64 per_cell = 2 * 19 * sb;
+    l.nc = bytes > l.planes_off ? ((bytes - l.planes_off) / per_cell) / 128 * 128 : 0;
+    return l;
+}
+
+// one wave per (row, segment): the mask of cells without the frozen bit and its popcount
+__global__ void __launch_bounds__(256) k_rc_masks(const std::uint32_t *__restrict__ links, RowSeg *__restrict__ rs,
+                                                  int nx, int nseg, i64 ents) {
+    const i64 w = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (w >= ents) return;  // wave-uniform
+    const unsigned lane = threadIdx.x & 63u;
+    const i64 row = w / nseg;
+    const int x = int(w - row * nseg) * 64 + int(lane);
+    const bool act = x < nx && !(links[row * nx + x] & FMX_LINK_FROZEN);
+    const unsigned long long m = __ballot(act);
+    if (lane == 0) {
+        rs[w].lo = unsigned(m);
+        rs[w].hi = unsigned(m >> 32);
+        rs[w].count = unsigned(__popcll(m));
+    }
+}
+
+// exclusive prefix sum of one value per thread over a block of 256; *total gets the block's sum
+__device__ unsigned rc_block_scan(unsigned v, unsigned *total) {
+    __shared__ unsigned wsum[4];
+    const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    unsigned inc = v;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const unsigned t = __shfl_up(inc, dd, 64);
+        if (lane >= unsigned(dd)) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    unsigned off = 0, tot = 0;
+#pragma unroll
+    for (unsigned i = 0; i < 4; ++i) {
+        off += i < w ? wsum[i] : 0u;
+        tot += wsum[i];
+    }
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ void __launch_bounds__(256) k_rc_chunk_sums(const RowSeg *__restrict__ rs, i64 ents,
+                                                       unsigned *__restrict__ part) {
+    const i64 e0 = i64(blockIdx.x) * kRcChunk + i64(threadIdx.x) * 16;
+    unsigned v = 0;
+    for (int i = 0; i < 16; ++i)
+        if (e0 + i < ents) v += rs[e0 + i].count;
+    unsigned tot;
+    rc_block_scan(v, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// one block: the chunk sums become their exclusive prefix; hdr[0] = all active cells
+__global__ void __launch_bounds__(256) k_rc_scan_chunks(unsigned *__restrict__ part, i64 chunks,
+                                                        unsigned long long *__restrict__ hdr) {
+    const i64 per = (chunks + 255) / 256;
+    const i64 c0 = i64(threadIdx.x) * per;
+    unsigned v = 0;
+    for (i64 i = 0; i < per; ++i)
+        if (c0 + i < chunks) v += part[c0 + i];
+    unsigned tot;
+    unsigned run = rc_block_scan(v, &tot);
+    for (i64 i = 0; i < per; ++i)
+        if (c0 + i < chunks) {
+            const unsigned t = part[c0 + i];
+            part[c0 + i] = run;
+            run += t;
+        }
+    if (threadIdx.x == 0) hdr[0] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_rc_bases(RowSeg *__restrict__ rs, i64 ents, const unsigned *__restrict__ part) {
+    const i64 e0 = i64(blockIdx.x) * kRcChunk + i64(threadIdx.x) * 16;
+    unsigned v = 0;
+    for (int i = 0; i < 16; ++i)
+        if (e0 + i < ents) v += rs[e0 + i].count;
+    unsigned tot;
+    unsigned run = part[blockIdx.x] + rc_block_scan(v, &tot);
+    for (int i = 0; i < 16; ++i)
+        if (e0 + i < ents) {
+            rs[e0 + i].base = run;
+            run += rs[e0 + i].count;
+        }
+}
+
+// Copies the 19 stored populations of every active cell between the box layout (plane stride n) and
+// the compact one (stride nc), without conversion.
+template <class S>
+__global__ void __launch_bounds__(256) k_rc_move(const S *__restrict__ src, S *__restrict__ dst,
+                                                 const RowSeg *__restrict__ rs, int nx, int nseg, i64 n, i64 nc,
+                                                 int to_compact) {
+    const i64 c = gtid();
+    if (c >= n) return;
+    const i64 row = c / nx;
+    const int x = int(c - row * nx);
+    const RowSeg e = rs[row * nseg + (x >> 6)];
+    const unsigned l = unsigned(x) & 63u;
+    const unsigned long long m = (static_cast<unsigned long long>(e.hi) << 32) | e.lo;
+    if (!((m >> l) & 1ull)) return;
+    const i64 k = i64(e.base) + __popcll(m & ((1ull << l) - 1ull));
+    const i64 si = to_compact ? c : k, sn = to_compact ? n : nc;
+    const i64 di = to_compact ? k : c, dn = to_compact ? nc : n;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) dst[q * dn + di] = src[q * sn + si];
+}
+
+// Builds the table from the current link words, checks that the workspace holds the active cells
+// (one host synchronisation) and copies f[cur] into compact buffer cur; *fits is false (and nothing
+// copied) when they do not fit, e.g. when fewer cells are frozen than when the workspace was sized.
+template <int P>
+static int rc_begin(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s, RcArgs &rc,
+                    void *cbuf[2], bool *fits) {
+    *fits = false;
+    using S = typename Storage<P>::S;
+    const RcLayout l = rc_layout(d, b.compact_work_bytes);
+    if (l.nc <= 0) return set_error(FMX_ERR_ARG, "fmx_coupled_run: compact_work_bytes %lld is too small",
+                                    (long long)b.compact_work_bytes);
+    char *w = static_cast<char *>(b.compact_work);
+    RowSeg *rs = reinterpret_cast<RowSeg *>(w);
+    unsigned *part = reinterpret_cast<unsigned *>(w + l.part_off);
+    unsigned long long *hdr = reinterpret_cast<unsigned long long *>(w + l.hdr_off);
+    const int nseg = (d.nx + 63) / 64;
+    const i64 n = i64(d.nx) * d.ny * d.nz;
+    hipLaunchKernelGGL(k_rc_masks, dim3(unsigned((l.ents + 3) / 4)), dim3(256), 0, s, b.links, rs, d.nx, nseg, l.ents);
+    if (int rc2 = after_launch("k_rc_masks")) return rc2;
+    hipLaunchKernelGGL(k_rc_chunk_sums, dim3(unsigned(l.chunks)), dim3(256), 0, s, rs, l.ents, part);
+    if (int rc2 = after_launch("k_rc_chunk_sums")) return rc2;
+    hipLaunchKernelGGL(k_rc_scan_chunks, dim3(1), dim3(256), 0, s, part, l.chunks, hdr);
+    if (int rc2 = after_launch("k_rc_scan_chunks")) return rc2;
+    hipLaunchKernelGGL(k_rc_bases, dim3(unsigned(l.chunks)), dim3(256), 0, s, rs, l.ents, part);
+    if (int rc2 = after_launch("k_rc_bases")) return rc2;
+    unsigned long long nact = 0;
+    if (int rc2 = check_hip(hipMemcpyAsync(&nact, hdr, sizeof(nact), hipMemcpyDeviceToHost, s),
+                            "fmx_coupled_run: hipMemcpyAsync"))
+        return rc2;
+    if (int rc2 = check_hip(hipStreamSynchronize(s), "fmx_coupled_run: hipStreamSynchronize")) return rc2;
+    if (i64(nact) > l.nc) return FMX_OK;
+    *fits = true;
+    cbuf[0] = w + l.planes_off;
+    cbuf[1] = w + l.planes_off + l.nc * 19 * i64(sizeof(S));
+    rc.rs = rs;
+    rc.nc = l.nc;
+    rc.nseg = nseg;
+    hipLaunchKernelGGL(k_rc_move<S>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
+                       static_cast<S *>(cbuf[cur]), rs, d.nx, nseg, n, l.nc, 1);
+    return after_launch("k_rc_move");
+}
+
 // Steps step0 .. step0+nsteps-1 on the ping-pong buffers; the outlet sums of step step0+k go to row
-// acc_row0 + k of outlet_acc.
+// acc_row0 + k of outlet_acc.  With a compact workspace (BGK, no mass log, at least kRcMinSteps
+// steps) the NS populations of the cells that are not frozen are stepped in row-compacted buffers and
+// copied back at the end; frozen cells keep their two alternating states in f[0] / f[1] as always.
 template <int P>
 int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
                       hipStream_t s, i64 acc_row0) {
     using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
     const CoupledParams<C> prm = make_params<C>(d);
     const bool ring = b.outlet_acc && b.outlet_ring;
     const i64 plane = i64(d.ny) * d.nz;
+    bool compact = b.compact_work && d.ns_op == FMX_NS_BGK && !b.mass_log && nsteps >= kRcMinSteps;
+    RcArgs rca{};
+    void *cbuf[2] = {nullptr, nullptr};
+    if (compact)
+        if (int rc = rc_begin<P>(d, b, cur, s, rca, cbuf, &compact)) return rc;
     int rows = 0;  // ring rows filled since the last fold
     for (int k = 0; k < nsteps; ++k) {
         CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
         if (k != nsteps - 1) ptr.inlet_raw = nullptr;  // the raw x = 0 plane is state read-back only
         if (ring) ptr.outlet_acc = b.outlet_ring + i64(rows) * plane;  // acc_row -1: store into this ring row
-        if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, s)) return rc;
+        if (compact) {
+            ptr.f = static_cast<const S *>(cbuf[(cur + k) & 1]);
+            ptr.fn = static_cast<S *>(cbuf[(cur + k + 1) & 1]);
+            if (int rc = launch_coupled_rc<P>(ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, rca, s)) return rc;
+        } else if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, s)) {
+            return rc;
+        }
         if (ring && (++rows == FMX_OUTLET_RING_ROWS || k == nsteps - 1)) {
             hipLaunchKernelGGL(k_outlet_ring_rows, dim3(rows), dim3(256), 0, s, b.outlet_ring, plane, b.outlet_acc,
                                acc_row0 + k + 1 - rows);
@@ -323,6 +519,13 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
                 return rc;
         }
     }
+    if (compact) {
+        const int last = (cur + nsteps) & 1;
+        const i64 n = prm.n;
+        hipLaunchKernelGGL(k_rc_move<S>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const S *>(cbuf[last]),
+                           static_cast<S *>(b.f[last]), rca.rs, d.nx, rca.nseg, n, rca.nc, 0);
+        if (int rc = after_launch("k_rc_move")) return rc;
+    }
     return FMX_OK;
 }
 
@@ -336,6 +539,16 @@ template int run_coupled_steps<FMX_BF16>(const fmx_coupled_desc &, const fmx_cou
                                          i64);
 
 template <int P>
+static int freeze_pass(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s, int allow) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    const CoupledParams<C> prm = make_params<C>(d);
+    hipLaunchKernelGGL(k_freeze<P>, dim3(blocks_for(prm.n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
+                       static_cast<S *>(b.f[1 - cur]), b.links, prm, allow);
+    return after_launch("k_freeze");
+}
+
+template <int P>
 static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
@@ -347,14 +560,22 @@ static int prologue(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
         if (int rc = after_launch("k_coupled_prologue")) return rc;
     }
     if (d.ns_op != FMX_NS_BGK) return FMX_OK;
-    // Opt-in (FMX_FREEZE=1): with the structure-of-arrays layout a frozen cell only saves traffic
-    // when whole cache lines are frozen, which dense sphere packs rarely give (measured slower on
-    // the 256x128x128 pack), so by default every solid cell stays on the full path.
+    // Opt-in (desc.freeze_solids, or FMX_FREEZE=1): frozen cells pay off in runs with a compact
+    // workspace, whose NS planes leave them out.  In the box layout a frozen cell only saves traffic
+    // when whole cache lines are frozen, which dense sphere packs rarely give (measured slower on the
+    // 256x128x128 pack).
     const char *fz = std::getenv("FMX_FREEZE");
-    const int allow = fz != nullptr && std::atoi(fz) != 0;
-    hipLaunchKernelGGL(k_freeze<P>, dim3(blocks_for(prm.n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
-                       static_cast<S *>(b.f[1 - cur]), b.links, prm, allow);
-    return after_launch("k_freeze");
+    return freeze_pass<P>(d, b, cur, s, (fz != nullptr && std::atoi(fz) != 0) || d.freeze_solids != 0);
+}
+
+int mark_frozen(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s) {
+    if (d.ns_op != FMX_NS_BGK || !d.freeze_solids) return clear_frozen(d, b.links, s);
+    switch (d.prec) {
+        case FMX_F32: return freeze_pass<FMX_F32>(d, b, cur, s, 1);
+        case FMX_F64: return freeze_pass<FMX_F64>(d, b, cur, s, 1);
+        case FMX_F16: return freeze_pass<FMX_F16>(d, b, cur, s, 1);
+        default: return freeze_pass<FMX_BF16>(d, b, cur, s, 1);
+    }
 }
 
 int clear_frozen(const fmx_coupled_desc &d, std::uint32_t *links, hipStream_t s) {
@@ -458,6 +679,10 @@ int fmx_coupled_validate(const fmx_coupled_desc *d, const fmx_coupled_bufs *b) {
     if (!b->f[0] || !b->f[1] || !b->g[0] || !b->g[1] || !b->q || !b->phi[0] || !b->phi[1] || !b->links)
         return set_error(FMX_ERR_ARG, "fmx_coupled: missing state buffer");
     if (d->ns_inlet && !b->inlet_raw) return set_error(FMX_ERR_ARG, "fmx_coupled: ns_inlet needs inlet_raw");
+    if (d->freeze_solids < 0 || d->freeze_solids > 1) return set_error(FMX_ERR_ARG, "fmx_coupled: freeze_solids is 0 or 1");
+    if (b->compact_work && b->compact_work_bytes < fmx_coupled_compact_bytes(d, 1))
+        return set_error(FMX_ERR_ARG, "fmx_coupled: compact_work_bytes %lld is below the row table's %lld",
+                         (long long)b->compact_work_bytes, (long long)fmx_coupled_compact_bytes(d, 1));
     if (d->ad_inlet == 2 && !b->inlet_series) return set_error(FMX_ERR_ARG, "fmx_coupled: ad_inlet 2 needs inlet_series");
     if (d->ns_op == FMX_NS_POROUS && !b->field)
         return FMX_OK;  // porosity defaults to 1 (PorousBGKCollision.cu:130)
@@ -575,6 +800,14 @@ int64_t fmx_coupled_recenter_workspace_bytes(const fmx_coupled_desc *d) {
     if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return -1;
     const i64 n = i64(d->nx) * d->ny * d->nz;
     return int64_t(mass_blocks(n) + 1) * kMassCols * int64_t(sizeof(double));
+}
+
+int64_t fmx_coupled_compact_bytes(const fmx_coupled_desc *d, int64_t active) {
+    if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || active < 0 || active > i64(d->nx) * d->ny * d->nz) return -1;
+    if (d->prec < FMX_F32 || d->prec > FMX_BF16) return -1;
+    const RcLayout l = rc_layout(*d, 0);
+    const i64 sb = d->prec == FMX_F64 ? 8 : (d->prec == FMX_F32 ? 4 : 2);
+    return l.planes_off + 2 * 19 * sb * ((active + 127) / 128 * 128);
 }
 
 int fmx_coupled_recenter(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, double target,

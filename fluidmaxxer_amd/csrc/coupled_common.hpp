@@ -282,6 +282,26 @@ int launch_coupled_step(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParam
 template <int P>
 int launch_coupled_pair(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm,
                         i64 step, i64 acc_row, hipStream_t s);
+// Row-compacted NS storage.  The NS planes hold only the cells that are not frozen, row (y, z) after
+// row in x order, so a run over a sphere pack moves no population of its frozen solid cells and every
+// cache line it touches is full.  One entry per (row, 64-cell segment): the segment's active-cell mask
+// and the compact index of its first active cell; the compact index of active cell x is base +
+// popcount(mask below x).
+struct RowSeg {
+    unsigned lo, hi;  // active mask of x = 64 seg .. 64 seg + 63 (bit x % 64)
+    unsigned base;    // compact index of the segment's first active cell
+    unsigned count;   // active cells in the segment
+};
+struct RcArgs {
+    const RowSeg *rs;  // (y + ny z) * nseg + seg
+    i64 nc;            // compact plane stride (>= active cells)
+    int nseg;          // segments per row, ceil(nx / 64)
+};
+// One step of the fused kernel on row-compacted NS planes (ptr.f / ptr.fn compact, everything else
+// in the box layout); BGK with frozen solid cells.
+template <int P>
+int launch_coupled_rc(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                      i64 acc_row, const RcArgs &rc, hipStream_t s);
 // nsteps fused steps on the ping-pong buffers of b, outlet sums of step step0+k into row acc_row0+k.
 template <int P>
 int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,
@@ -292,5 +312,8 @@ int check_run_buffers(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
                       bool writes_outlet = true);
 // Clears the frozen-solid bits of the link words (every cell back on the full update path).
 int clear_frozen(const fmx_coupled_desc &d, std::uint32_t *links, hipStream_t s);
+// Marks the frozen solid cells of state f[cur] when desc.freeze_solids asks for them (BGK), writing
+// their next phase into f[1 - cur]; otherwise clears the frozen bits.
+int mark_frozen(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s);
 
 }  // namespace fmx

@@ -13,6 +13,10 @@ template int launch_coupled_step<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, con
                                           hipStream_t);
 template int launch_coupled_step<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
                                            hipStream_t);
+template int launch_coupled_rc<FMX_F16>(const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
+                                        const RcArgs &, hipStream_t);
+template int launch_coupled_rc<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
+                                         const RcArgs &, hipStream_t);
 template int launch_coupled_pair<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
                                           hipStream_t);
 template int launch_coupled_pair<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,

@@ -19,9 +19,12 @@ namespace fmx {
 // looked at, and the pulls are not wrapped in a branch that is decided lane by lane.  (Ignoring the bit
 // is always correct: a frozen cell's two buffers hold its two alternating states, which the full path
 // reproduces.)
-template <int P, int OP, bool WIDE, bool FZ>
-__global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P == FMX_F64 ? 4 : 1)) k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step,
-                                                 i64 acc_row) {
+// RC: row-compacted NS planes (RowSeg; implies FZ): p.f / p.fn hold only the active cells; a frozen
+// lane stores no NS population and hands zero velocity to the AD update.
+template <int P, int OP, bool WIDE, bool FZ, bool RC>
+__device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const CoupledParams<typename Storage<P>::C> &prm,
+                                             i64 step, i64 acc_row, const RcArgs &rc) {
+    static_assert(!RC || (FZ && OP == FMX_NS_BGK), "row-compacted storage is the frozen-solid BGK step");
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
     using Off = std::conditional_t<WIDE, i64, unsigned>;
@@ -38,7 +41,6 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
     const i64 n = prm.n;
     const Off cell = Off(x) + Off(nx) * (Off(y) + Off(ny) * Off(z));
     const Off cell_b = cell * SB;
-    const Off plane_b = Off(n) * SB;
     const std::uint32_t lk = ld_b(p.links, cell * 4u);
     const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
     // a frozen solid cell alternates between the two states held in f and f_new and has zero
@@ -80,25 +82,70 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
         if (cx == 0) return cell_b + d;
         return cell_b + (((cx == 1) ? xm_ok : xp_ok) ? d : Off(0));
     };
+    // Row-compacted NS planes: the compact index of x in row (cy, cz) counts that row's active cells
+    // before x (mbcnt of the segment's mask, which this wave's 64 lanes cover exactly, plus the segment
+    // base); x - 1 is one less and x + 1 adds the row's bit at x, also across the segment's ends.  A
+    // pull whose source is solid never uses its index (the bounce-back offset replaces it), so only
+    // the indices of active sources matter.  Keep-local pulls use this cell's own index.
+    Off rsl[3][3] = {};
+    unsigned rbit[3][3] = {};
+    if constexpr (RC) {
+        const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
+        const unsigned lane = unsigned(x) & 63u;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int yy = rowok[a][b] ? y + dy[a] : y, zz = rowok[a][b] ? z + dz[b] : z;
+                const RowSeg &e = rc.rs[(i64(yy) + i64(ny) * zz) * rc.nseg + seg];
+                const unsigned lo = __builtin_amdgcn_readfirstlane(e.lo), hi = __builtin_amdgcn_readfirstlane(e.hi);
+                const unsigned base = __builtin_amdgcn_readfirstlane(e.base);
+                rsl[a][b] = Off(__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, base)));
+                rbit[a][b] = ((lane < 32u ? (lo >> lane) : (hi >> (lane - 32u))) & 1u);
+            }
+    }
+    const Off ncell = RC ? rsl[1][1] : cell;  // this cell's index in the NS planes
+    const Off ncell_b = ncell * SB;
+    const i64 nn = RC ? rc.nc : n;  // NS plane stride
+    const Off nplane_b = Off(nn) * SB;
+    auto src_n = [&](int cx, int cy, int cz) -> Off {  // byte offset of the NS source cell within a plane
+        if constexpr (!RC) {
+            return src_b(cx, cy, cz);
+        } else {
+            bool ok = rowok[cy + 1][cz + 1];
+            Off si = rsl[cy + 1][cz + 1];
+            if (cx == 1) {
+                ok = ok && xm_ok;
+                si = si - Off(1);
+            } else if (cx == -1) {
+                ok = ok && xp_ok;
+                si = si + Off(rbit[cy + 1][cz + 1]);
+            }
+            return (ok ? si : ncell) * Off(SB);
+        }
+    };
 
     // ---------------------------------------------------------------- NS pull
     // With bounce-back links, direction q and its opposite are addressed from the lower of the two
     // planes, so the bounce-back pull (this cell along opp(q)) and the streaming pull differ only in
     // the offset, chosen by the link bit.
+    // RC: a frozen lane loads too (no branch around the run of loads): every link of a solid cell
+    // bounces back, so its offsets are its compact index -- the next active cell of its row, whose
+    // lines the wave reads anyway -- and its results are discarded (zero velocity, no stores).
     C fo[19];
-    if (!frozen) static_for<0, 19>([&](auto qc) {
+    if (RC || !frozen) static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
         constexpr int o = Q19::opp[q];
         constexpr bool BB = OP == FMX_NS_BGK && q != 0;
         constexpr int bq = BB ? (q < o ? q : o) : q;
-        Off off = src_b(cx, cy, cz) + Off(q - bq) * plane_b;
+        Off off = src_n(cx, cy, cz) + Off(q - bq) * nplane_b;
         if constexpr (BB) {
-            const Off boff = cell_b + Off(o - bq) * plane_b;
+            const Off boff = ncell_b + Off(o - bq) * nplane_b;
             const Off m = Off(int(lk << (31 - q)) >> 31);  // all ones where the q link bounces back
             off = (boff & m) | (off & ~m);
         }
-        fo[q] = decode<P>(ld_b(plane(p.f, bq, n), off), w19<C>(q), prm.f_ref);
+        fo[q] = decode<P>(ld_b(plane(p.f, bq, nn), off), w19<C>(q), prm.f_ref);
     });
 
     // ------------------------------------------- AD pull and sorption inputs
@@ -134,10 +181,16 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
 
     // ------------------------------------------ NS collide (in place) and moments
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
-    if (!frozen) {
+    if (RC || !frozen) {
         const C fld = p.field ? ld_b(p.field, cell * CB) : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
         ns_collide_cell<OP, IsHalf<P>::value>(fo, solid, fld, prm);
         post_moments<IsHalf<P>::value>(fo, r2, ux2, uy2, uz2);
+        if constexpr (RC) {
+            r2 = frozen ? C(0) : r2;
+            ux2 = frozen ? C(0) : ux2;
+            uy2 = frozen ? C(0) : uy2;
+            uz2 = frozen ? C(0) : uz2;
+        }
     }
 
     // ------------------------------------------------------------- NS stores
@@ -174,7 +227,7 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
             C v = fo[q];
             if (inlet_cell) v = (prm.ns_inlet == 2) ? inlet_eq<q>(prm) : inlet_neq<q>(fp[q], rp, up_x, up_y, up_z, prm);
             else if (outlet_cell) v = outlet_pressure<q>(um_x, um_y, um_z, prm);
-            st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+            st_plane_nt(plane(p.fn, q, nn), ncell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
         });
     } else {
         // the x = 1 thread writes the NEQ inlet of x = 0, the x = nx-2 thread the pressure outlet
@@ -182,25 +235,25 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
             if (prm.ns_inlet == 2) {  // next step's VelocityDirichletEq: f(0,y,z) = feq(rho0, u_BC)
                 static_for<0, 19>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
+                    st_plane_nt(plane(p.fn, q, nn), ncell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
                 });
             }
         } else if (!outlet_cell && !frozen) {
 #pragma unroll
             for (int q = 0; q < 19; ++q)
-                st_plane_nt(plane(p.fn, q, n), cell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
+                st_plane_nt(plane(p.fn, q, nn), ncell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
         }
         if (prm.ns_inlet == 1 && x == 1) {  // next step's NEQ update of (0, y, z)
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                st_b(plane(p.fn, q, n), cell_b - SB,
+                st_b(plane(p.fn, q, nn), ncell_b - SB,  // x = 0 and 1 are never frozen: adjacent in compact rows too
                      encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
             });
         }
         if (prm.ns_outlet && x == nx - 2) {  // PressureDirichlet of (nx-1, y, z)
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                st_b(plane(p.fn, q, n), cell_b + SB,
+                st_b(plane(p.fn, q, nn), ncell_b + SB,  // nor x = nx-2 and nx-1
                      encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
             });
         }
@@ -269,6 +322,32 @@ __global__ void __launch_bounds__(256, (P == FMX_F16 || P == FMX_BF16) ? 7 : (P 
                             double(phi));
         }
     }
+}
+
+constexpr int coupled_waves(int P) { return (P == FMX_F16 || P == FMX_BF16) ? 7 : (P == FMX_F64 ? 4 : 1); }
+
+template <int P, int OP, bool WIDE, bool FZ>
+__global__ void __launch_bounds__(256, coupled_waves(P))
+    k_coupled(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step, i64 acc_row) {
+    coupled_body<P, OP, WIDE, FZ, false>(p, prm, step, acc_row, RcArgs{});
+}
+
+template <int P, bool WIDE>
+__global__ void __launch_bounds__(256, coupled_waves(P))
+    k_coupled_rc(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step, i64 acc_row, RcArgs rc) {
+    coupled_body<P, FMX_NS_BGK, WIDE, true, true>(p, prm, step, acc_row, rc);
+}
+
+template <int P>
+int launch_coupled_rc(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                      i64 acc_row, const RcArgs &rc, hipStream_t s) {
+    const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
+    // compact planes are no larger than the box's, and the box's AD planes decide as in k_coupled
+    const char *fw = std::getenv("FMX_WIDE");
+    const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(typename Storage<P>::S)) > (i64(1) << 32);
+    if (wide) hipLaunchKernelGGL((k_coupled_rc<P, true>), grid, block, 0, s, ptr, prm, step, acc_row, rc);
+    else hipLaunchKernelGGL((k_coupled_rc<P, false>), grid, block, 0, s, ptr, prm, step, acc_row, rc);
+    return after_launch("k_coupled_rc");
 }
 
 template <int P>

@@ -31,8 +31,10 @@ extern "C" {
  * non-finite count, and its periodic log inside fmx_coupled_run (mass_log).
  * 3: FP32 checkpoints of an FP64 run, whose reverse sweep runs in FP32 (fmx_adjoint_bufs field_rev,
  * tau_field_rev)
- * 4: fmx_coupled_recenter (the density reference of 16-bit NS storage moved to the flow's mean) */
-#define FMX_ABI_VERSION 4
+ * 4: fmx_coupled_recenter (the density reference of 16-bit NS storage moved to the flow's mean)
+ * 5: fmx_coupled_desc::freeze_solids, fmx_coupled_bufs::compact_work, fmx_coupled_compact_bytes
+ *    (row-compacted NS storage that leaves frozen solid cells out of a run) */
+#define FMX_ABI_VERSION 5
 
 /* storage precision (DATA_TYPE, Types.hpp:6, plus half-precision deviation storage) */
 enum { FMX_F32 = 0, FMX_F64 = 1, FMX_F16 = 2, FMX_BF16 = 3 };
@@ -180,6 +182,11 @@ typedef struct fmx_coupled_desc {
                                        it); half storage of g holds g = w * (phi_src + s * h),
                                        s = max(phi_src, 2^-14), phi_src the source cell's lagged
                                        concentration (the x = nx-2 value on a Neumann outlet plane) */
+    int32_t freeze_solids;          /* BGK: 1 = the prologue and fmx_checkpoint_load mark frozen solid cells
+                                       (FMX_LINK_FROZEN: a solid cell at rest away from the x boundary
+                                       planes, whose state only alternates), so that runs given a compact
+                                       workspace skip them */
+    int32_t reserved0;
 } fmx_coupled_desc;
 
 #define FMX_OUTLET_SLOTS 64
@@ -215,6 +222,14 @@ typedef struct fmx_coupled_bufs {
     int64_t mass_log_rows;    /* rows of mass_log (>= nsteps / mass_every) */
     int64_t mass_workspace_bytes;
     int64_t mass_every;       /* steps between mass_log rows (>= 1 with mass_log) */
+    void *compact_work;       /* nullable, BGK: workspace of fmx_coupled_compact_bytes(desc, active) bytes; a
+                                 run then steps the NS populations of the cells without FMX_LINK_FROZEN in
+                                 row-compacted buffers there (the frozen ones are never read or written) and
+                                 leaves f[] as without it, to the bit.  Ignored with mass_log, by runs of
+                                 fewer than 16 steps and when the active cells do not fit (those runs use
+                                 the box layout).  A run given it synchronises the stream once, after
+                                 building its row table */
+    int64_t compact_work_bytes;
 } fmx_coupled_bufs;
 #define FMX_OUTLET_RING_ROWS 64
 
@@ -258,6 +273,11 @@ int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *buf
  * the state is unchanged and *f_ref_out is still the mean (or target).  The reference must stay within
  * a factor of two of desc->f_ref (FMX_ERR_ARG otherwise). */
 int64_t fmx_coupled_recenter_workspace_bytes(const fmx_coupled_desc *desc);
+/* No counterpart in the reference (its BGK kernel reads and writes every solid cell).  Bytes of
+ * fmx_coupled_bufs::compact_work for a run in which `active` cells lack FMX_LINK_FROZEN (count them in
+ * the link words after the prologue): a row table of 16 bytes per 64 cells of a row, and two compact
+ * copies of the 19 population planes of the active cells. */
+int64_t fmx_coupled_compact_bytes(const fmx_coupled_desc *desc, int64_t active);
 int fmx_coupled_recenter(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, double target,
                          double *f_ref_out, void *workspace, int64_t workspace_bytes, void *stream);
 /* OutletCSV::log average (OutletCSV.hpp:51-80) for nsteps steps from outlet_acc: mean[k] = sum / count */
@@ -340,8 +360,9 @@ int fmx_adjoint_backward(const fmx_coupled_desc *desc, const fmx_coupled_bufs *b
  * ckpt (slots of fmx_adjoint_ckpt_bytes(desc, ckpt_prec)) <- the fused state f[cur], g[cur], q,
  * phi[cur], as stored, (FP32 run) as FP16 / BF16 deviations or (FP64 run) rounded to FP32.
  * fmx_checkpoint_load puts it back (a 16-bit slot decoded to FP32, an FP32 slot widened to FP64) and
- * clears the frozen-solid link bits, so fmx_coupled_run goes on
- * from it as from the step it was saved before (exactly, for a checkpoint at the storage precision). */
+ * marks the frozen solid cells again from it (desc->freeze_solids; else it clears their link bits), so
+ * fmx_coupled_run goes on from it as from the step it was saved before (exactly, for a checkpoint at
+ * the storage precision). */
 int fmx_checkpoint_save(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, void *ckpt, int ckpt_prec,
                         int64_t slot, void *stream);
 int fmx_checkpoint_load(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, const void *ckpt,

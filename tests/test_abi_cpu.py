@@ -20,7 +20,7 @@ def test_library_exports_every_header_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(names) == set(abi._SIGS), set(names) ^ set(abi._SIGS)
-    assert L.fmx_abi_version() == 4 == abi.ABI_VERSION
+    assert L.fmx_abi_version() == 5 == abi.ABI_VERSION
 
 
 def test_validate_rejects_bad_descriptors():
@@ -44,6 +44,38 @@ def test_validate_rejects_bad_descriptors():
     assert abi.lib().fmx_coupled_validate(C.byref(d), C.byref(b)) == 1
     with pytest.raises(abi.FmxError):
         abi.call("fmx_coupled_run", C.byref(d), C.byref(b), 1, 1, 0, None)
+
+
+def test_compact_workspace_size_and_validation():
+    """fmx_coupled_compact_bytes: a row table of 16 bytes per (row, 64-cell segment), 256-byte aligned,
+    the scan's chunk sums and a 256-byte header, then two compact copies of the 19 NS planes of the
+    active cells rounded up to 128 cells; out-of-range arguments give -1.  A workspace below the
+    table's size is refused by validation (no GPU needed)."""
+    L = abi.lib()
+    n = 130 * 6 * 5
+    cfg = CoupledConfig(130, 6, 5, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
+    ents = 6 * 5 * 3  # rows x ceil(130 / 64)
+    head = (ents * 16 + 255) // 256 * 256 + 256 + 256  # table, chunk sums (one chunk), header
+    for prec, sb in ((abi.F32, 4), (abi.F64, 8), (abi.F16, 2), (abi.BF16, 2)):
+        d = cfg.desc(prec)
+        for active in (0, 1, 128, 129, n):
+            want = head + 2 * 19 * sb * ((active + 127) // 128 * 128)
+            assert L.fmx_coupled_compact_bytes(C.byref(d), active) == want, (prec, active)
+        assert L.fmx_coupled_compact_bytes(C.byref(d), -1) == -1
+        assert L.fmx_coupled_compact_bytes(C.byref(d), n + 1) == -1
+    d = cfg.desc(abi.F32)
+    b = abi.CoupledBufs()
+    for i in range(2):
+        b.f[i] = b.g[i] = b.phi[i] = 0x1000
+    b.q = b.links = b.inlet_raw = 0x1000
+    assert L.fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.OK
+    b.compact_work, b.compact_work_bytes = 0x1000, L.fmx_coupled_compact_bytes(C.byref(d), 1) - 1
+    assert L.fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.ERR_ARG
+    assert b"compact_work_bytes" in L.fmx_last_error()
+    b.compact_work_bytes += 1
+    assert L.fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.OK
+    d.freeze_solids = 2
+    assert L.fmx_coupled_validate(C.byref(d), C.byref(b)) == abi.ERR_ARG
 
 
 def test_recenter_refuses_a_far_reference_before_any_launch():

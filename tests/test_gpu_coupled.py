@@ -346,12 +346,12 @@ def test_link_words_match_host_rule(periodic):
     region = (rng.random(n) < 0.5).astype(np.uint8)
     cfg = CoupledConfig(nx, ny, nz, solid=solid, region=region, periodic_y=periodic[0], periodic_z=periodic[1])
     say(f"links periodic={periodic}")
-    sim = CoupledSim(cfg, prec="f32")
+    sim = CoupledSim(cfg, prec="f32", compact=False)
     raw = host(sim.links.view(torch_int32())).astype(np.int64) & 0xFFFFFFFF
     frozen = (raw >> 21) & 1
     links = raw & ~(1 << 21)
     x, y, z = coords(nx, ny, nz)
-    assert not frozen.any()  # freezing is opt-in (FMX_FREEZE=1)
+    assert not frozen.any()  # freezing is opt-in (compact runs, FMX_FREEZE=1)
     expect = np.where(region == 1, 1 << 19, 0) | np.where(solid == 1, 1 << 20, 0)
     for q in range(1, 19):
         xs, ys, zs = x - CX19[q], y - CY19[q], z - CZ19[q]
