@@ -70,12 +70,24 @@ public:
         fmx::freeBytes(d_acc_);
         fmx::freeBytes(d_mean_);
         fmx::freeBytes(d_ring_);
+        fmx::freeBytes(d_compact_);
     }
     CoupledStep(const CoupledStep &) = delete;
     CoupledStep &operator=(const CoupledStep &) = delete;
 
     bool fused() const { return fused_; }
     const char *whyNotFused() const { return why_; }
+
+    // No counterpart in the reference (whose BGK kernel updates every solid cell).  Fused BGK runs of
+    // 16 steps or more leave the solid cells at rest out of the NS traffic: they are frozen at the
+    // prologue (their state only alternates) and the other cells' NS populations are stepped in
+    // row-compacted buffers, with the same results.  Takes effect from the next run().
+    void useCompactSolids(bool on = true) {
+        finish();
+        compact_ = on && desc_.ns_op == FMX_NS_BGK;
+        desc_.freeze_solids = compact_ ? 1 : 0;
+        primed_ = false;  // the next run marks (or clears) the frozen cells
+    }
 
     // one step
     void step(int s) { run(s, 1); }
@@ -94,6 +106,7 @@ public:
         if (!primed_) {
             fmx::check(fmx_coupled_prologue(&desc_, bufsFor(nullptr), 0, nullptr), "fmx_coupled_prologue");
             primed_ = true;
+            if (compact_) sizeCompact();
         }
         double *acc = nullptr;
         if (logOutlet) acc = accFor(n);
@@ -257,12 +270,32 @@ private:
         bufs_.inlet_series_len = d_series_ ? series_cap_ : 0;
         bufs_.outlet_acc = acc;
         bufs_.outlet_acc_rows = acc ? acc_cap_ : 0;
+        if (compact_ && d_compact_) {
+            bufs_.compact_work = d_compact_;
+            bufs_.compact_work_bytes = compact_bytes_;
+        }
         if (acc) {  // breakthrough sums in a fixed order (the same curve on every run)
             if (!d_ring_)
                 d_ring_ = static_cast<double *>(fmx::mallocBytes(sizeof(double) * FMX_OUTLET_RING_ROWS * nyz_));
             bufs_.outlet_ring = d_ring_;
         }
         return &bufs_;
+    }
+
+    // workspace of the row-compacted runs, sized for the cells the prologue left unfrozen
+    void sizeCompact() {
+        const std::size_t n = std::size_t(desc_.nx) * std::size_t(desc_.ny) * std::size_t(desc_.nz);
+        std::vector<std::uint32_t> links(n);
+        fmx::memcpyDtoH(links.data(), d_links_, sizeof(std::uint32_t) * n);
+        std::int64_t active = 0;
+        for (std::uint32_t w : links) active += (w & FMX_LINK_FROZEN) ? 0 : 1;
+        const std::int64_t bytes = fmx_coupled_compact_bytes(&desc_, active);
+        if (bytes < 0) throw std::runtime_error("FMX error: fmx_coupled_compact_bytes refused the configuration");
+        if (bytes > compact_bytes_) {
+            fmx::freeBytes(d_compact_);
+            d_compact_ = fmx::mallocBytes(std::size_t(bytes));
+            compact_bytes_ = bytes;
+        }
     }
 
     void ensureSeries(int last_step) {
@@ -332,7 +365,9 @@ private:
     std::vector<std::uint8_t> region_;
     std::unique_ptr<RegionBuf> d_region_;
     std::size_t nyz_ = 0;
-    bool fused_ = false, primed_ = false;
+    bool fused_ = false, primed_ = false, compact_ = false;
+    void *d_compact_ = nullptr;
+    std::int64_t compact_bytes_ = 0;
     const char *why_ = "not analysed";
     fmx_coupled_desc desc_{};
     fmx_coupled_bufs bufs_{};

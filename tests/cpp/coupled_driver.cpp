@@ -97,6 +97,7 @@ int main(int argc, char **argv) {
         else cs.reset(new CoupledStep<D3Q19, D3Q7>(ns, ad, q, lin, reg.mask));
         wasFused = cs->fused();
         if (!wasFused) std::fprintf(stderr, "not fused: %s\n", cs->whyNotFused());
+        if (kv["compact"] == "1") cs->useCompactSolids();
         const int split = std::atoi(kv["split"].c_str());
         if (split > 0 && split < steps) {  // two calls: the second grows the cached per-step inlet values
             cs->run(1, split, true);

@@ -70,6 +70,21 @@ def test_cpp_api_matches_oracle(tmp_path, dtype, oprec, tol, inlet, iso):
         assert all(v <= tol for v in errs.values()), (mode, errs)
 
 
+@pytest.mark.parametrize("dtype", ["float", "double"])
+def test_cpp_api_compact_solids_same_bits(tmp_path, dtype):
+    """CoupledStep::useCompactSolids(): the fused runs leave the solid cells at rest out of the NS
+    traffic (row-compacted planes) and give the same fields and outlet curve, to the bit, as the fused
+    runs without it; split into two runs (each at least 16 steps, so both are compact)."""
+    cfg = _config("neq", "langmuir")
+    runs = {}
+    for extra in ((), ("compact=1",)):
+        got, info = _run_driver(tmp_path, cfg, dtype, "fused", 120, "neq", "langmuir", ("split=50", *extra))
+        assert "fused=1" in info, info
+        runs[extra] = got
+    for k, v in runs[()].items():
+        assert np.array_equal(v, runs[("compact=1",)][k]), k
+
+
 PULSE = (0.1, 0.9, 300, 900, 40.0)  # base, amp, t_on, t_off, k: the run straddles both edges
 
 
