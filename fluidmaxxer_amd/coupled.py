@@ -97,9 +97,9 @@ class CoupledSim:
     float64 for f64 and float32 otherwise).
     compact: BGK only -- solid cells at rest are frozen at the prologue and runs of 16 steps or more
     step the NS populations of the other cells in row-compacted buffers (a workspace sized for them),
-    so the frozen ones cost no population traffic; results are the same to the bit.  Default (None):
-    on for FP32 / FP64 storage (512^3 sphere pack: 1.24-1.30x the box layout), off for FP16 / BF16
-    (no faster: their step is not bound by the bytes it moves).
+    so the frozen ones cost no population traffic; results are the same to the bit (FP32 / FP64; 16-bit
+    storage runs a split NS / AD step there, the same values to the storage rounding).  Default (None):
+    on (512^3 sphere pack: FP32 1.24-1.30x the box layout, FP64 1.29x, FP16 / BF16 1.07x).
     """
 
     def __init__(self, cfg: CoupledConfig, prec: str = "f32", device: str = "cuda:0", compact: bool = None):
@@ -132,7 +132,7 @@ class CoupledSim:
             self.inlet_series = torch.as_tensor(np.asarray(cfg.inlet_series, np.float64)).to(dev)
         self._desc = cfg.desc(self.prec)
         if compact is None:
-            compact = self.prec in (abi.F32, abi.F64)
+            compact = True
         self.compact = bool(compact) and cfg.ns_op == "bgk"
         self._desc.freeze_solids = int(self.compact)
         self._compact_ws = None

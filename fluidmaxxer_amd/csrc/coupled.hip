@@ -301,11 +301,17 @@ extern template int launch_coupled_rc<FMX_F16>(const CoupledPtrs<FMX_F16> &, con
                                                const RcArgs &, hipStream_t);
 extern template int launch_coupled_rc<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
                                                 const RcArgs &, hipStream_t);
+extern template int launch_coupled_split<FMX_F16>(const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
+                                                  const SplitArgs &, hipStream_t);
+extern template int launch_coupled_split<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64,
+                                                   i64, const SplitArgs &, hipStream_t);
 
 // ------------------------------------------------------------ row-compacted NS storage
 // Workspace (fmx_coupled_bufs::compact_work): the RowSeg table, the scan's chunk sums, a header whose
 // first word is the number of active cells, then the two compact population buffers of 19 planes
-// each, whose plane stride nc is what the remaining bytes hold (a multiple of 128 cells).
+// each, whose plane stride nc is what the remaining bytes hold (a multiple of 128 cells); with 16-bit
+// storage (the split step) also the box index and link word (8 bytes) and velocity (3 FP32) of each
+// active cell.
 constexpr int kRcChunk = 4096;  // table entries per block of the scan (256 threads x 16)
 // Shorter runs stay in the box layout: building the table, the host synchronisation and the two
 // copies cost about two compact steps per run.
@@ -324,9 +330,23 @@ static RcLayout rc_layout(const fmx_coupled_desc &d, i64 bytes) {
     l.hdr_off = l.part_off + align256(l.chunks * 4);
     l.planes_off = l.hdr_off + 256;
     const i64 sb = d.prec == FMX_F64 ? 8 : (d.prec == FMX_F32 ? 4 : 2);
-    const i64 per_cell = 2 * 19 * sb;
+    const i64 per_cell = 2 * 19 * sb + (sb == 2 ? 8 + 3 * 4 : 0);
     l.nc = bytes > l.planes_off ? ((bytes - l.planes_off) / per_cell) / 128 * 128 : 0;
     return l;
+}
+
+// box index and link word of every active cell at its compact index (the split step's NS threads)
+__global__ void __launch_bounds__(256) k_rc_coords(const RowSeg *__restrict__ rs, const std::uint32_t *__restrict__ links,
+                                                   uint2 *__restrict__ coords, int nx, int nseg, i64 n) {
+    const i64 c = gtid();
+    if (c >= n) return;
+    const i64 row = c / nx;
+    const int x = int(c - row * nx);
+    const RowSeg e = rs[row * nseg + (x >> 6)];
+    const unsigned l = unsigned(x) & 63u;
+    const unsigned long long m = (static_cast<unsigned long long>(e.hi) << 32) | e.lo;
+    if (!((m >> l) & 1ull)) return;
+    coords[i64(e.base) + __popcll(m & ((1ull << l) - 1ull))] = make_uint2(unsigned(c), links[c]);
 }
 
 // one wave per (row, segment): the mask of cells without the frozen bit and its popcount
@@ -438,7 +458,7 @@ __global__ void __launch_bounds__(256) k_rc_move(const S *__restrict__ src, S *_
 // copied) when they do not fit, e.g. when fewer cells are frozen than when the workspace was sized.
 template <int P>
 static int rc_begin(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cur, hipStream_t s, RcArgs &rc,
-                    void *cbuf[2], bool *fits) {
+                    void *cbuf[2], bool *fits, SplitArgs *sa) {
     *fits = false;
     using S = typename Storage<P>::S;
     const RcLayout l = rc_layout(d, b.compact_work_bytes);
@@ -470,6 +490,17 @@ static int rc_begin(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
     rc.rs = rs;
     rc.nc = l.nc;
     rc.nseg = nseg;
+    if (sa) {  // the split step: box index and velocity of every active cell after the two buffers
+        uint2 *coords = reinterpret_cast<uint2 *>(w + l.planes_off + 2 * l.nc * 19 * i64(sizeof(S)));
+        sa->rs = rs;
+        sa->nc = l.nc;
+        sa->nseg = nseg;
+        sa->coords = coords;
+        sa->nact = i64(nact);
+        sa->u = coords + l.nc;
+        hipLaunchKernelGGL(k_rc_coords, dim3(blocks_for(n)), dim3(256), 0, s, rs, b.links, coords, d.nx, nseg, n);
+        if (int rc2 = after_launch("k_rc_coords")) return rc2;
+    }
     hipLaunchKernelGGL(k_rc_move<S>, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<const S *>(b.f[cur]),
                        static_cast<S *>(cbuf[cur]), rs, d.nx, nseg, n, l.nc, 1);
     return after_launch("k_rc_move");
@@ -488,10 +519,14 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
     const bool ring = b.outlet_acc && b.outlet_ring;
     const i64 plane = i64(d.ny) * d.nz;
     bool compact = b.compact_work && d.ns_op == FMX_NS_BGK && !b.mass_log && nsteps >= kRcMinSteps;
+    // 16-bit storage runs the split step in the compact layout (FMX_SPLIT=0: the fused compact kernel)
+    const char *sp = std::getenv("FMX_SPLIT");
+    const bool split = IsHalf<P>::value && !(sp && std::atoi(sp) == 0);
     RcArgs rca{};
+    SplitArgs sa{};
     void *cbuf[2] = {nullptr, nullptr};
     if (compact)
-        if (int rc = rc_begin<P>(d, b, cur, s, rca, cbuf, &compact)) return rc;
+        if (int rc = rc_begin<P>(d, b, cur, s, rca, cbuf, &compact, split ? &sa : nullptr)) return rc;
     int rows = 0;  // ring rows filled since the last fold
     for (int k = 0; k < nsteps; ++k) {
         CoupledPtrs<P> ptr = make_ptrs<P>(b, (cur + k) & 1);
@@ -500,7 +535,14 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
         if (compact) {
             ptr.f = static_cast<const S *>(cbuf[(cur + k) & 1]);
             ptr.fn = static_cast<S *>(cbuf[(cur + k + 1) & 1]);
-            if (int rc = launch_coupled_rc<P>(ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, rca, s)) return rc;
+            const i64 row = ring ? -1 : acc_row0 + k;
+            int rc;
+            if constexpr (IsHalf<P>::value)
+                rc = split ? launch_coupled_split<P>(ptr, prm, step0 + k, row, sa, s)
+                           : launch_coupled_rc<P>(ptr, prm, step0 + k, row, rca, s);
+            else
+                rc = launch_coupled_rc<P>(ptr, prm, step0 + k, row, rca, s);
+            if (rc) return rc;
         } else if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, s)) {
             return rc;
         }
@@ -807,7 +849,8 @@ int64_t fmx_coupled_compact_bytes(const fmx_coupled_desc *d, int64_t active) {
     if (d->prec < FMX_F32 || d->prec > FMX_BF16) return -1;
     const RcLayout l = rc_layout(*d, 0);
     const i64 sb = d->prec == FMX_F64 ? 8 : (d->prec == FMX_F32 ? 4 : 2);
-    return l.planes_off + 2 * 19 * sb * ((active + 127) / 128 * 128);
+    const i64 per_cell = 2 * 19 * sb + (sb == 2 ? 8 + 3 * 4 : 0);
+    return l.planes_off + per_cell * ((active + 127) / 128 * 128);
 }
 
 int fmx_coupled_recenter(const fmx_coupled_desc *d, const fmx_coupled_bufs *b, int cur, double target,

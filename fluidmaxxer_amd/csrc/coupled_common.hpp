@@ -287,7 +287,7 @@ int launch_coupled_pair(int ns_op, const CoupledPtrs<P> &ptr, const CoupledParam
 // cache line it touches is full.  One entry per (row, 64-cell segment): the segment's active-cell mask
 // and the compact index of its first active cell; the compact index of active cell x is base +
 // popcount(mask below x).
-struct RowSeg {
+struct alignas(16) RowSeg {
     unsigned lo, hi;  // active mask of x = 64 seg .. 64 seg + 63 (bit x % 64)
     unsigned base;    // compact index of the segment's first active cell
     unsigned count;   // active cells in the segment
@@ -302,6 +302,21 @@ struct RcArgs {
 template <int P>
 int launch_coupled_rc(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
                       i64 acc_row, const RcArgs &rc, hipStream_t s);
+// The split step (16-bit storage on row-compacted NS planes): an NS kernel whose waves hold only active
+// cells (their box index and link word in `coords`) stores each active cell's velocity in u, then an AD kernel over
+// the box reads it.  The 16-bit step is bound by its memory instructions, and the NS ones are then
+// issued for the active cells only.
+struct SplitArgs {
+    const RowSeg *rs;
+    i64 nc;                  // compact plane stride
+    int nseg;
+    const uint2 *coords;     // box index and link word of active cell k
+    i64 nact;
+    void *u;                 // ux, uy, uz of active cell k at k, nc + k, 2 nc + k (compute precision)
+};
+template <int P>
+int launch_coupled_split(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                         i64 acc_row, const SplitArgs &sa, hipStream_t s);
 // nsteps fused steps on the ping-pong buffers of b, outlet sums of step step0+k into row acc_row0+k.
 template <int P>
 int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 step0, int nsteps, int cur,

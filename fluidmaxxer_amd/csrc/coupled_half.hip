@@ -17,6 +17,10 @@ template int launch_coupled_rc<FMX_F16>(const CoupledPtrs<FMX_F16> &, const Coup
                                         const RcArgs &, hipStream_t);
 template int launch_coupled_rc<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
                                          const RcArgs &, hipStream_t);
+template int launch_coupled_split<FMX_F16>(const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
+                                           const SplitArgs &, hipStream_t);
+template int launch_coupled_split<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
+                                            const SplitArgs &, hipStream_t);
 template int launch_coupled_pair<FMX_F16>(int, const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
                                           hipStream_t);
 template int launch_coupled_pair<FMX_BF16>(int, const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,

@@ -21,10 +21,13 @@ namespace fmx {
 // reproduces.)
 // RC: row-compacted NS planes (RowSeg; implies FZ): p.f / p.fn hold only the active cells; a frozen
 // lane stores no NS population and hands zero velocity to the AD update.
-template <int P, int OP, bool WIDE, bool FZ, bool RC>
+// ADONLY: the AD half of the split step (XA = SplitArgs): no NS work; the velocity of an active cell
+// comes from the NS kernel's u, a frozen cell's is zero.
+template <int P, int OP, bool WIDE, bool FZ, bool RC, bool ADONLY = false, class XA = RcArgs>
 __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const CoupledParams<typename Storage<P>::C> &prm,
-                                             i64 step, i64 acc_row, const RcArgs &rc) {
+                                             i64 step, i64 acc_row, const XA &rc) {
     static_assert(!RC || (FZ && OP == FMX_NS_BGK), "row-compacted storage is the frozen-solid BGK step");
+    static_assert(!ADONLY || (FZ && !RC && OP == FMX_NS_BGK), "the AD half of the split step");
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
     using Off = std::conditional_t<WIDE, i64, unsigned>;
@@ -89,6 +92,14 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     // the indices of active sources matter.  Keep-local pulls use this cell's own index.
     Off rsl[3][3] = {};
     unsigned rbit[3][3] = {};
+    Off uidx = 0;  // ADONLY: compact index of this cell (of the next active one in its row if frozen)
+    if constexpr (ADONLY) {
+        const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
+        const RowSeg &e = rc.rs[(i64(y) + i64(ny) * z) * rc.nseg + seg];
+        const unsigned lo = __builtin_amdgcn_readfirstlane(e.lo), hi = __builtin_amdgcn_readfirstlane(e.hi);
+        const unsigned base = __builtin_amdgcn_readfirstlane(e.base);
+        uidx = Off(__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, base)));
+    }
     if constexpr (RC) {
         const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
         const unsigned lane = unsigned(x) & 63u;
@@ -133,7 +144,7 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     // bounces back, so its offsets are its compact index -- the next active cell of its row, whose
     // lines the wave reads anyway -- and its results are discarded (zero velocity, no stores).
     C fo[19];
-    if (RC || !frozen) static_for<0, 19>([&](auto qc) {
+    if (!ADONLY && (RC || !frozen)) static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
         constexpr int o = Q19::opp[q];
@@ -181,7 +192,15 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
 
     // ------------------------------------------ NS collide (in place) and moments
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
-    if (RC || !frozen) {
+    if constexpr (ADONLY) {
+        if (prm.coupling) {  // every lane loads (its index is in the box of active cells); frozen: zero
+            const C *u = static_cast<const C *>(rc.u);
+            const Off nub = Off(rc.nc) * CB;
+            ux2 = frozen ? C(0) : ld_b(u, uidx * CB);
+            uy2 = frozen ? C(0) : ld_b(u, nub + uidx * CB);
+            uz2 = frozen ? C(0) : ld_b(u, nub + nub + uidx * CB);
+        }
+    } else if (RC || !frozen) {
         const C fld = p.field ? ld_b(p.field, cell * CB) : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
         ns_collide_cell<OP, IsHalf<P>::value>(fo, solid, fld, prm);
         post_moments<IsHalf<P>::value>(fo, r2, ux2, uy2, uz2);
@@ -196,6 +215,7 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     // ------------------------------------------------------------- NS stores
     const bool inlet_cell = prm.ns_inlet && x == 0;
     const bool outlet_cell = prm.ns_outlet && x == nx - 1;
+    if constexpr (!ADONLY) {
     if (inlet_cell && p.inlet_raw) {  // the x = 0 populations as the reference holds them (last step of a run)
         const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
 #pragma unroll
@@ -258,6 +278,7 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
             });
         }
     }
+    }  // !ADONLY
 
     // ------------------------------------------------------ sorption source and AD collision
     C src_term = C(0);
@@ -336,6 +357,162 @@ template <int P, bool WIDE>
 __global__ void __launch_bounds__(256, coupled_waves(P))
     k_coupled_rc(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step, i64 acc_row, RcArgs rc) {
     coupled_body<P, FMX_NS_BGK, WIDE, true, true>(p, prm, step, acc_row, rc);
+}
+
+// The NS half of the split step: one thread per active cell (compact index k; box index and link
+// word in coords[k]),
+// the NS pulls, collision and stores of k_coupled_rc with the boundary writes of its
+// neighbouring-thread form (the x = 0 / 1 and nx-2 / nx-1 cells are never frozen, so they are
+// adjacent in the compact order too), and the cell's velocity stored for the AD half.
+template <int P, bool WIDE>
+__global__ void __launch_bounds__(256, coupled_waves(P))
+    k_ns_split(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, SplitArgs sa) {
+    using C = typename Storage<P>::C;
+    using S = typename Storage<P>::S;
+    using Off = std::conditional_t<WIDE, i64, unsigned>;
+    constexpr unsigned SB = sizeof(S);
+    const unsigned total = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (prm.xcd_remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
+    const i64 k = i64(b) * 256 + threadIdx.x;
+    if (k >= sa.nact) return;
+    const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
+    const uint2 cl = sa.coords[k];
+    const unsigned c = cl.x;
+    const int x = int(c % unsigned(nx));
+    const unsigned t = c / unsigned(nx);
+    const int y = int(t % unsigned(ny)), z = int(t / unsigned(ny));
+    const std::uint32_t lk = cl.y;
+    const bool solid = lk & FMX_LINK_SOLID;
+    const bool xm_ok = x > 0, xp_ok = x < nx - 1;
+    int dy[3] = {1, 0, -1}, dz[3] = {1, 0, -1};
+    bool yin[3] = {y + 1 < ny, true, y > 0}, zin[3] = {z + 1 < nz, true, z > 0};
+    if (prm.py) {
+        dy[0] = yin[0] ? 1 : 1 - ny;
+        dy[2] = yin[2] ? -1 : ny - 1;
+        yin[0] = yin[2] = true;
+    }
+    if (prm.pz) {
+        dz[0] = zin[0] ? 1 : 1 - nz;
+        dz[2] = zin[2] ? -1 : nz - 1;
+        zin[0] = zin[2] = true;
+    }
+    // compact index of x in each neighbouring row (its segment's base plus the popcount of its mask
+    // below x) and the row's active bit at x; this row's is k itself
+    const int seg = x >> 6;
+    const unsigned jl = unsigned(x) & 63u;
+    const unsigned long long below = (1ull << jl) - 1ull;
+    Off rsl[3][3];
+    unsigned rbit[3][3];
+    bool rowok[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            rowok[a][bb] = yin[a] && zin[bb];
+            if (a == 1 && bb == 1) {
+                rsl[a][bb] = Off(k);
+                rbit[a][bb] = 1u;
+                continue;
+            }
+            const int yy = rowok[a][bb] ? y + dy[a] : y, zz = rowok[a][bb] ? z + dz[bb] : z;
+            const RowSeg e = sa.rs[(i64(yy) + i64(ny) * zz) * sa.nseg + seg];
+            const unsigned long long m = (static_cast<unsigned long long>(e.hi) << 32) | e.lo;
+            rsl[a][bb] = Off(e.base + unsigned(__builtin_popcountll(m & below)));
+            rbit[a][bb] = unsigned(m >> jl) & 1u;
+        }
+    const Off ncell_b = Off(k) * SB;
+    const Off nplane_b = Off(sa.nc) * SB;
+    auto src_n = [&](int cx, int cy, int cz) -> Off {
+        bool ok = rowok[cy + 1][cz + 1];
+        Off si = rsl[cy + 1][cz + 1];
+        if (cx == 1) {
+            ok = ok && xm_ok;
+            si = si - Off(1);
+        } else if (cx == -1) {
+            ok = ok && xp_ok;
+            si = si + Off(rbit[cy + 1][cz + 1]);
+        }
+        return (ok ? si : Off(k)) * Off(SB);
+    };
+    C fo[19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Q19::cx[q], cy = Q19::cy[q], cz = Q19::cz[q];
+        constexpr int o = Q19::opp[q];
+        constexpr bool BB = q != 0;
+        constexpr int bq = BB ? (q < o ? q : o) : q;
+        Off off = src_n(cx, cy, cz) + Off(q - bq) * nplane_b;
+        if constexpr (BB) {
+            const Off boff = ncell_b + Off(o - bq) * nplane_b;
+            const Off m = Off(int(lk << (31 - q)) >> 31);
+            off = (boff & m) | (off & ~m);
+        }
+        fo[q] = decode<P>(ld_b(plane(p.f, bq, sa.nc), off), w19<C>(q), prm.f_ref);
+    });
+    C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
+    ns_collide_cell<FMX_NS_BGK, IsHalf<P>::value>(fo, solid, C(0), prm);
+    post_moments<IsHalf<P>::value>(fo, r2, ux2, uy2, uz2);
+    C *u = static_cast<C *>(sa.u);
+    u[k] = ux2;
+    u[sa.nc + k] = uy2;
+    u[2 * sa.nc + k] = uz2;
+    const bool inlet_cell = prm.ns_inlet && x == 0;
+    const bool outlet_cell = prm.ns_outlet && x == nx - 1;
+    if (inlet_cell && p.inlet_raw) {
+        const unsigned np = unsigned(ny) * unsigned(nz), pc = unsigned(y) + unsigned(ny) * unsigned(z);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
+    }
+    if (inlet_cell) {
+        if (prm.ns_inlet == 2) {
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                st_plane_nt(plane(p.fn, q, sa.nc), ncell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
+            });
+        }
+    } else if (!outlet_cell) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q)
+            st_plane_nt(plane(p.fn, q, sa.nc), ncell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
+    }
+    if (prm.ns_inlet == 1 && x == 1) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            st_b(plane(p.fn, q, sa.nc), ncell_b - SB,
+                 encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
+        });
+    }
+    if (prm.ns_outlet && x == nx - 2) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            st_b(plane(p.fn, q, sa.nc), ncell_b + SB,
+                 encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
+        });
+    }
+}
+
+template <int P, bool WIDE>
+__global__ void __launch_bounds__(256, coupled_waves(P))
+    k_ad_split(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, i64 step, i64 acc_row, SplitArgs sa) {
+    coupled_body<P, FMX_NS_BGK, WIDE, true, false, true, SplitArgs>(p, prm, step, acc_row, sa);
+}
+
+template <int P>
+int launch_coupled_split(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,
+                         i64 acc_row, const SplitArgs &sa, hipStream_t s) {
+    const char *fw = std::getenv("FMX_WIDE");
+    const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(typename Storage<P>::S)) > (i64(1) << 32);
+    if (sa.nact > 0) {
+        const dim3 g1(unsigned((sa.nact + 255) / 256)), b1(256);
+        if (wide) hipLaunchKernelGGL((k_ns_split<P, true>), g1, b1, 0, s, ptr, prm, sa);
+        else hipLaunchKernelGGL((k_ns_split<P, false>), g1, b1, 0, s, ptr, prm, sa);
+        if (int rc = after_launch("k_ns_split")) return rc;
+    }
+    const dim3 grid(coupled_blocks(prm.gx, prm.gy, prm.gz)), block(prm.block_x, prm.block_y, 1);
+    if (wide) hipLaunchKernelGGL((k_ad_split<P, true>), grid, block, 0, s, ptr, prm, step, acc_row, sa);
+    else hipLaunchKernelGGL((k_ad_split<P, false>), grid, block, 0, s, ptr, prm, step, acc_row, sa);
+    return after_launch("k_ad_split");
 }
 
 template <int P>
