@@ -359,68 +359,23 @@ __global__ void __launch_bounds__(256, coupled_waves(P))
     coupled_body<P, FMX_NS_BGK, WIDE, true, true>(p, prm, step, acc_row, rc);
 }
 
-// The NS half of the split step: one thread per active cell (compact index k; box index and link
-// word in coords[k]),
-// the NS pulls, collision and stores of k_coupled_rc with the boundary writes of its
-// neighbouring-thread form (the x = 0 / 1 and nx-2 / nx-1 cells are never frozen, so they are
-// adjacent in the compact order too), and the cell's velocity stored for the AD half.
+// The NS work of active cell k (compact index) at (x, y, z) with link word lk, given the compact index
+// of x in each neighbouring row (rsl; this row's is k), that row's active bit at x and whether the row
+// exists (keep-local otherwise): the NS pulls, collision and stores of k_coupled_rc with the boundary
+// writes of its neighbouring-thread form (the x = 0 / 1 and nx-2 / nx-1 cells are never frozen, so
+// they are adjacent in the compact order too), and the cell's velocity stored for the AD half.
 template <int P, bool WIDE>
-__global__ void __launch_bounds__(256, coupled_waves(P))
-    k_ns_split(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, SplitArgs sa) {
+__device__ __forceinline__ void ns_split_cell(const CoupledPtrs<P> &p, const CoupledParams<typename Storage<P>::C> &prm,
+                                              const SplitArgs &sa, i64 k, int x, int y, int z, std::uint32_t lk,
+                                              const std::conditional_t<WIDE, i64, unsigned> (&rsl)[3][3],
+                                              const unsigned (&rbit)[3][3], const bool (&rowok)[3][3]) {
     using C = typename Storage<P>::C;
     using S = typename Storage<P>::S;
     using Off = std::conditional_t<WIDE, i64, unsigned>;
     constexpr unsigned SB = sizeof(S);
-    const unsigned total = gridDim.x;
-    unsigned b = blockIdx.x;
-    if (prm.xcd_remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
-    const i64 k = i64(b) * 256 + threadIdx.x;
-    if (k >= sa.nact) return;
     const int nx = prm.nx, ny = prm.ny, nz = prm.nz;
-    const uint2 cl = sa.coords[k];
-    const unsigned c = cl.x;
-    const int x = int(c % unsigned(nx));
-    const unsigned t = c / unsigned(nx);
-    const int y = int(t % unsigned(ny)), z = int(t / unsigned(ny));
-    const std::uint32_t lk = cl.y;
     const bool solid = lk & FMX_LINK_SOLID;
     const bool xm_ok = x > 0, xp_ok = x < nx - 1;
-    int dy[3] = {1, 0, -1}, dz[3] = {1, 0, -1};
-    bool yin[3] = {y + 1 < ny, true, y > 0}, zin[3] = {z + 1 < nz, true, z > 0};
-    if (prm.py) {
-        dy[0] = yin[0] ? 1 : 1 - ny;
-        dy[2] = yin[2] ? -1 : ny - 1;
-        yin[0] = yin[2] = true;
-    }
-    if (prm.pz) {
-        dz[0] = zin[0] ? 1 : 1 - nz;
-        dz[2] = zin[2] ? -1 : nz - 1;
-        zin[0] = zin[2] = true;
-    }
-    // compact index of x in each neighbouring row (its segment's base plus the popcount of its mask
-    // below x) and the row's active bit at x; this row's is k itself
-    const int seg = x >> 6;
-    const unsigned jl = unsigned(x) & 63u;
-    const unsigned long long below = (1ull << jl) - 1ull;
-    Off rsl[3][3];
-    unsigned rbit[3][3];
-    bool rowok[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-            rowok[a][bb] = yin[a] && zin[bb];
-            if (a == 1 && bb == 1) {
-                rsl[a][bb] = Off(k);
-                rbit[a][bb] = 1u;
-                continue;
-            }
-            const int yy = rowok[a][bb] ? y + dy[a] : y, zz = rowok[a][bb] ? z + dz[bb] : z;
-            const RowSeg e = sa.rs[(i64(yy) + i64(ny) * zz) * sa.nseg + seg];
-            const unsigned long long m = (static_cast<unsigned long long>(e.hi) << 32) | e.lo;
-            rsl[a][bb] = Off(e.base + unsigned(__builtin_popcountll(m & below)));
-            rbit[a][bb] = unsigned(m >> jl) & 1u;
-        }
     const Off ncell_b = Off(k) * SB;
     const Off nplane_b = Off(sa.nc) * SB;
     auto src_n = [&](int cx, int cy, int cz) -> Off {
@@ -490,6 +445,78 @@ __global__ void __launch_bounds__(256, coupled_waves(P))
                  encode<P>(outlet_pressure<q>(ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
         });
     }
+}
+
+// Rows (y - cy, z - cz) around (y, z): the row deltas with periodic wrap, and whether each
+// row exists (a = cy + 1, b = cz + 1).
+template <class C>
+__device__ __forceinline__ void neighbour_rows(const CoupledParams<C> &prm, int y, int z, int (&dy)[3], int (&dz)[3],
+                                               bool (&rowok)[3][3]) {
+    const int ny = prm.ny, nz = prm.nz;
+    bool yin[3] = {y + 1 < ny, true, y > 0}, zin[3] = {z + 1 < nz, true, z > 0};
+    dy[0] = 1;
+    dy[1] = 0;
+    dy[2] = -1;
+    dz[0] = 1;
+    dz[1] = 0;
+    dz[2] = -1;
+    if (prm.py) {
+        dy[0] = yin[0] ? 1 : 1 - ny;
+        dy[2] = yin[2] ? -1 : ny - 1;
+        yin[0] = yin[2] = true;
+    }
+    if (prm.pz) {
+        dz[0] = zin[0] ? 1 : 1 - nz;
+        dz[2] = zin[2] ? -1 : nz - 1;
+        zin[0] = zin[2] = true;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) rowok[a][b] = yin[a] && zin[b];
+}
+
+// The NS half of the split step, one thread per active cell in compact order (box index and link word
+// in coords[k]; the neighbouring rows' table entries gathered per lane, from L1 / L2).
+template <int P, bool WIDE>
+__global__ void __launch_bounds__(256, coupled_waves(P))
+    k_ns_split(CoupledPtrs<P> p, CoupledParams<typename Storage<P>::C> prm, SplitArgs sa) {
+    using Off = std::conditional_t<WIDE, i64, unsigned>;
+    const unsigned total = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (prm.xcd_remap && (total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);
+    const i64 k = i64(b) * 256 + threadIdx.x;
+    if (k >= sa.nact) return;
+    const int nx = prm.nx, ny = prm.ny;
+    const uint2 cl = sa.coords[k];
+    const unsigned c = cl.x;
+    const int x = int(c % unsigned(nx));
+    const unsigned t = c / unsigned(nx);
+    const int y = int(t % unsigned(ny)), z = int(t / unsigned(ny));
+    int dy[3], dz[3];
+    bool rowok[3][3];
+    neighbour_rows(prm, y, z, dy, dz, rowok);
+    const int seg = x >> 6;
+    const unsigned jl = unsigned(x) & 63u;
+    const unsigned long long below = (1ull << jl) - 1ull;
+    Off rsl[3][3];
+    unsigned rbit[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            if (a == 1 && bb == 1) {
+                rsl[a][bb] = Off(k);
+                rbit[a][bb] = 1u;
+                continue;
+            }
+            const int yy = rowok[a][bb] ? y + dy[a] : y, zz = rowok[a][bb] ? z + dz[bb] : z;
+            const RowSeg e = sa.rs[(i64(yy) + i64(ny) * zz) * sa.nseg + seg];
+            const unsigned long long m = (static_cast<unsigned long long>(e.hi) << 32) | e.lo;
+            rsl[a][bb] = Off(e.base + unsigned(__builtin_popcountll(m & below)));
+            rbit[a][bb] = unsigned(m >> jl) & 1u;
+        }
+    ns_split_cell<P, WIDE>(p, prm, sa, k, x, y, z, cl.y, rsl, rbit, rowok);
 }
 
 template <int P, bool WIDE>
