@@ -41,15 +41,22 @@ B_FIELD = {"f32": 4, "f64": 8, "f16": 4, "bf16": 4}  # per-cell porosity / solid
 B_NS = {"f32": 152, "f64": 304, "f16": 76, "bf16": 76}  # of B_ALG: the 19 NS populations read and written
 
 
+def split_step():
+    """Whether compact runs take the split step (the library's default; FMX_SPLIT=0 turns it off)."""
+    return os.environ.get("FMX_SPLIT", "1").strip() not in ("0",)
+
+
 def compact_bytes(sim, prec):
     """Algorithmic bytes per cell update of a run in row-compacted NS planes: the NS populations of the
-    active cells only (the frozen solid cells' are never moved), the rest as B_ALG; and the active
-    fraction."""
+    active cells only (the frozen solid cells' are never moved), the rest as B_ALG, and with the split
+    step each active cell's 8-byte record read and its velocity (three values of the compute precision)
+    written and read; and the active fraction."""
     from fluidmaxxer_amd import abi
     n = sim.cfg.n
     active = int((sim.links & abi.LINK_FROZEN).eq(0).sum().item())
     a = active / n
-    return B_ALG[prec] - B_NS[prec] * (1.0 - a), a
+    extra = a * (8 + 2 * 3 * (8 if prec == "f64" else 4)) if split_step() else 0.0
+    return B_ALG[prec] - B_NS[prec] * (1.0 - a) + extra, a
 
 
 def cpu_info():
@@ -579,15 +586,17 @@ def main():
             "traffic_source": (f"{traffic_src}: rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this kernel and workload, "
                                "scaled by a plain copy's reading; not measured in this run") if traffic_src else None,
             "bytes_per_cell_update": b_alg,
-            "kernel": (("k_ns_split + k_ad_split (row-compacted NS planes; two launches per step" if args.prec in
-                        ("f16", "bf16") else "k_coupled_rc (row-compacted NS planes; one launch per step") +
+            "kernel": (("k_ns_split + k_ad_split (row-compacted NS planes: NS over the active cells, then AD over "
+                        "the box; two launches per step" if split_step() else
+                        "k_coupled_rc (row-compacted NS planes; one launch per step") +
                        ", plus one row-table build and one copy in and out per run)") if compact
                       else "k_coupled (one launch per step)",
         },
     }
     if compact:
         # the same step time over the bytes the row-compacted layout moves: the frozen solid cells' NS
-        # populations (B_NS of each) are neither read nor written
+        # populations (B_NS of each) are neither read nor written; the split step hands each active
+        # cell's velocity over in memory
         line["roofline"]["compact_layout"] = {
             "active_fraction": round(active, 4),
             "bytes_per_cell_update": round(b_moved, 1),

@@ -97,9 +97,10 @@ class CoupledSim:
     float64 for f64 and float32 otherwise).
     compact: BGK only -- solid cells at rest are frozen at the prologue and runs of 16 steps or more
     step the NS populations of the other cells in row-compacted buffers (a workspace sized for them),
-    so the frozen ones cost no population traffic; results are the same to the bit (FP32 / FP64; 16-bit
-    storage runs a split NS / AD step there, the same values to the storage rounding).  Default (None):
-    on (512^3 sphere pack: FP32 1.24-1.30x the box layout, FP64 1.29x, FP16 / BF16 1.07x).
+    so the frozen ones cost no population traffic, each step being an NS launch over those cells and an
+    AD launch over the box; results are the same to the bit (FP32 / FP64; 16-bit storage to its
+    rounding, the kernels being compiled with FMA contraction).  Default (None): on (512^3 sphere pack:
+    FP32 1.28x the box layout, FP64 1.31x, FP16 / BF16 1.07-1.13x).
     """
 
     def __init__(self, cfg: CoupledConfig, prec: str = "f32", device: str = "cuda:0", compact: bool = None):

@@ -301,6 +301,10 @@ extern template int launch_coupled_rc<FMX_F16>(const CoupledPtrs<FMX_F16> &, con
                                                const RcArgs &, hipStream_t);
 extern template int launch_coupled_rc<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64, i64,
                                                 const RcArgs &, hipStream_t);
+template int launch_coupled_split<FMX_F32>(const CoupledPtrs<FMX_F32> &, const CoupledParams<float> &, i64, i64,
+                                           const SplitArgs &, hipStream_t);
+template int launch_coupled_split<FMX_F64>(const CoupledPtrs<FMX_F64> &, const CoupledParams<double> &, i64, i64,
+                                           const SplitArgs &, hipStream_t);
 extern template int launch_coupled_split<FMX_F16>(const CoupledPtrs<FMX_F16> &, const CoupledParams<float> &, i64, i64,
                                                   const SplitArgs &, hipStream_t);
 extern template int launch_coupled_split<FMX_BF16>(const CoupledPtrs<FMX_BF16> &, const CoupledParams<float> &, i64,
@@ -310,8 +314,8 @@ extern template int launch_coupled_split<FMX_BF16>(const CoupledPtrs<FMX_BF16> &
 // Workspace (fmx_coupled_bufs::compact_work): the RowSeg table, the scan's chunk sums, a header whose
 // first word is the number of active cells, then the two compact population buffers of 19 planes
 // each, whose plane stride nc is what the remaining bytes hold (a multiple of 128 cells); with 16-bit
-// storage (the split step) also the box index and link word (8 bytes) and velocity (3 FP32) of each
-// active cell.
+// storage (the split step) also the box index and link word (8 bytes) and velocity (three values of
+// the compute precision) of each active cell.
 constexpr int kRcChunk = 4096;  // table entries per block of the scan (256 threads x 16)
 // Shorter runs stay in the box layout: building the table, the host synchronisation and the two
 // copies cost about two compact steps per run.
@@ -320,6 +324,15 @@ constexpr int kRcMinSteps = 16;
 struct RcLayout {
     i64 ents, chunks, part_off, hdr_off, planes_off, nc;
 };
+// Compact runs take the split step (FMX_SPLIT=0: the fused compact kernel instead); the workspace then
+// holds the per-active-cell record and velocity as well
+static bool split_step(int) {
+    const char *sp = std::getenv("FMX_SPLIT");
+    return !(sp && std::atoi(sp) == 0);
+}
+// the split step's bytes per active cell beyond the NS planes: box index and link word (uint2), then
+// the velocity, three values of the compute precision (FP64 for FP64 storage, FP32 otherwise)
+static i64 split_cell_bytes(int prec) { return 8 + 3 * (prec == FMX_F64 ? 8 : 4); }
 static i64 align256(i64 v) { return (v + 255) & ~i64(255); }
 static RcLayout rc_layout(const fmx_coupled_desc &d, i64 bytes) {
     RcLayout l{};
@@ -330,7 +343,7 @@ static RcLayout rc_layout(const fmx_coupled_desc &d, i64 bytes) {
     l.hdr_off = l.part_off + align256(l.chunks * 4);
     l.planes_off = l.hdr_off + 256;
     const i64 sb = d.prec == FMX_F64 ? 8 : (d.prec == FMX_F32 ? 4 : 2);
-    const i64 per_cell = 2 * 19 * sb + (sb == 2 ? 8 + 3 * 4 : 0);
+    const i64 per_cell = 2 * 19 * sb + (split_step(d.prec) ? split_cell_bytes(d.prec) : 0);
     l.nc = bytes > l.planes_off ? ((bytes - l.planes_off) / per_cell) / 128 * 128 : 0;
     return l;
 }
@@ -519,9 +532,7 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
     const bool ring = b.outlet_acc && b.outlet_ring;
     const i64 plane = i64(d.ny) * d.nz;
     bool compact = b.compact_work && d.ns_op == FMX_NS_BGK && !b.mass_log && nsteps >= kRcMinSteps;
-    // 16-bit storage runs the split step in the compact layout (FMX_SPLIT=0: the fused compact kernel)
-    const char *sp = std::getenv("FMX_SPLIT");
-    const bool split = IsHalf<P>::value && !(sp && std::atoi(sp) == 0);
+    const bool split = split_step(d.prec);
     RcArgs rca{};
     SplitArgs sa{};
     void *cbuf[2] = {nullptr, nullptr};
@@ -536,13 +547,9 @@ int run_coupled_steps(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, i64 
             ptr.f = static_cast<const S *>(cbuf[(cur + k) & 1]);
             ptr.fn = static_cast<S *>(cbuf[(cur + k + 1) & 1]);
             const i64 row = ring ? -1 : acc_row0 + k;
-            int rc;
-            if constexpr (IsHalf<P>::value)
-                rc = split ? launch_coupled_split<P>(ptr, prm, step0 + k, row, sa, s)
-                           : launch_coupled_rc<P>(ptr, prm, step0 + k, row, rca, s);
-            else
-                rc = launch_coupled_rc<P>(ptr, prm, step0 + k, row, rca, s);
-            if (rc) return rc;
+            if (int rc = split ? launch_coupled_split<P>(ptr, prm, step0 + k, row, sa, s)
+                               : launch_coupled_rc<P>(ptr, prm, step0 + k, row, rca, s))
+                return rc;
         } else if (int rc = launch_coupled_step<P>(d.ns_op, ptr, prm, step0 + k, ring ? -1 : acc_row0 + k, s)) {
             return rc;
         }
@@ -849,7 +856,7 @@ int64_t fmx_coupled_compact_bytes(const fmx_coupled_desc *d, int64_t active) {
     if (d->prec < FMX_F32 || d->prec > FMX_BF16) return -1;
     const RcLayout l = rc_layout(*d, 0);
     const i64 sb = d->prec == FMX_F64 ? 8 : (d->prec == FMX_F32 ? 4 : 2);
-    const i64 per_cell = 2 * 19 * sb + (sb == 2 ? 8 + 3 * 4 : 0);
+    const i64 per_cell = 2 * 19 * sb + (split_step(d->prec) ? split_cell_bytes(d->prec) : 0);
     return l.planes_off + per_cell * ((active + 127) / 128 * 128);
 }
 

@@ -224,8 +224,9 @@ typedef struct fmx_coupled_bufs {
     int64_t mass_every;       /* steps between mass_log rows (>= 1 with mass_log) */
     void *compact_work;       /* nullable, BGK: workspace of fmx_coupled_compact_bytes(desc, active) bytes; a
                                  run then steps the NS populations of the cells without FMX_LINK_FROZEN in
-                                 row-compacted buffers there (the frozen ones are never read or written) and
-                                 leaves f[] as without it, to the bit.  Ignored with mass_log, by runs of
+                                 row-compacted buffers there (the frozen ones are never read or written), each
+                                 step an NS launch over those cells and an AD launch over the box (FMX_SPLIT=0:
+                                 one fused launch), and leaves f[] as without it, to the bit.  Ignored with mass_log, by runs of
                                  fewer than 16 steps and when the active cells do not fit (those runs use
                                  the box layout).  A run given it synchronises the stream once, after
                                  building its row table */
@@ -275,8 +276,9 @@ int fmx_coupled_view_g(const fmx_coupled_desc *desc, const fmx_coupled_bufs *buf
 int64_t fmx_coupled_recenter_workspace_bytes(const fmx_coupled_desc *desc);
 /* No counterpart in the reference (its BGK kernel reads and writes every solid cell).  Bytes of
  * fmx_coupled_bufs::compact_work for a run in which `active` cells lack FMX_LINK_FROZEN (count them in
- * the link words after the prologue): a row table of 16 bytes per 64 cells of a row, and two compact
- * copies of the 19 population planes of the active cells. */
+ * the link words after the prologue): a row table of 16 bytes per 64 cells of a row, two compact
+ * copies of the 19 population planes of the active cells, and per active cell an 8-byte record and a
+ * velocity of three compute-precision values (none of the last two with FMX_SPLIT=0). */
 int64_t fmx_coupled_compact_bytes(const fmx_coupled_desc *desc, int64_t active);
 int fmx_coupled_recenter(const fmx_coupled_desc *desc, const fmx_coupled_bufs *bufs, int cur, double target,
                          double *f_ref_out, void *workspace, int64_t workspace_bytes, void *stream);

@@ -49,15 +49,15 @@ def test_validate_rejects_bad_descriptors():
 def test_compact_workspace_size_and_validation():
     """fmx_coupled_compact_bytes: a row table of 16 bytes per (row, 64-cell segment), 256-byte aligned,
     the scan's chunk sums and a 256-byte header, then two compact copies of the 19 NS planes of the
-    active cells rounded up to 128 cells (16-bit storage: also an 8-byte box index and link word and
-    an FP32 velocity per active cell, for the split step); out-of-range arguments give -1.  A workspace below the
+    active cells rounded up to 128 cells, and for the split step an 8-byte box index and link word
+    and a velocity (three values of the compute precision) per active cell; out-of-range arguments give -1.  A workspace below the
     table's size is refused by validation (no GPU needed)."""
     L = abi.lib()
     n = 130 * 6 * 5
     cfg = CoupledConfig(130, 6, 5, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
     ents = 6 * 5 * 3  # rows x ceil(130 / 64)
     head = (ents * 16 + 255) // 256 * 256 + 256 + 256  # table, chunk sums (one chunk), header
-    for prec, per in ((abi.F32, 2 * 19 * 4), (abi.F64, 2 * 19 * 8), (abi.F16, 2 * 19 * 2 + 8 + 12),
+    for prec, per in ((abi.F32, 2 * 19 * 4 + 8 + 12), (abi.F64, 2 * 19 * 8 + 8 + 24), (abi.F16, 2 * 19 * 2 + 8 + 12),
                       (abi.BF16, 2 * 19 * 2 + 8 + 12)):
         d = cfg.desc(prec)
         for active in (0, 1, 128, 129, n):
@@ -65,6 +65,14 @@ def test_compact_workspace_size_and_validation():
             assert L.fmx_coupled_compact_bytes(C.byref(d), active) == want, (prec, active)
         assert L.fmx_coupled_compact_bytes(C.byref(d), -1) == -1
         assert L.fmx_coupled_compact_bytes(C.byref(d), n + 1) == -1
+    # FMX_SPLIT=0 (the fused compact kernel): the NS planes only
+    os.environ["FMX_SPLIT"] = "0"
+    try:
+        for prec, per in ((abi.F64, 2 * 19 * 8), (abi.F32, 2 * 19 * 4), (abi.F16, 2 * 19 * 2)):
+            d = cfg.desc(prec)
+            assert L.fmx_coupled_compact_bytes(C.byref(d), 200) == head + per * 256, prec
+    finally:
+        del os.environ["FMX_SPLIT"]
     d = cfg.desc(abi.F32)
     b = abi.CoupledBufs()
     for i in range(2):

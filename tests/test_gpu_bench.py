@@ -34,7 +34,7 @@ def test_bench_line_contract():
     # FP32 runs in row-compacted NS planes by default: the counter summary of that kernel, and the same
     # time over the bytes the compact run moves
     assert rf["traffic"] > 0 and rf["traffic_source"].startswith("profiles/traffic_rc_c2_f32.json")
-    assert d["config"]["ns_layout"].startswith("row-compacted") and rf["kernel"].startswith("k_coupled_rc")
+    assert d["config"]["ns_layout"].startswith("row-compacted") and rf["kernel"].startswith("k_ns_split + k_ad_split")
     cl = rf["compact_layout"]
     assert 0.3 < cl["active_fraction"] < 0.5 and cl["bytes_per_cell_update"] < rf["bytes_per_cell_update"]
     assert abs(cl["frac"] / rf["frac"] - cl["bytes_per_cell_update"] / rf["bytes_per_cell_update"]) < 1e-3
