@@ -85,6 +85,26 @@ def test_compact_run_bit_identical(prec, case):
     assert_same(dense, comp)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64", "f16"])
+@pytest.mark.parametrize("case", ["neq_24", "split_boundary_65", "wide_256"])
+def test_compact_fused_kernel(prec, case, monkeypatch):
+    """FMX_SPLIT=0: compact runs take the fused compact kernel (one launch per step, k_coupled_rc)
+    instead of the split step; the same state as the box layout (FP16: to its rounding)."""
+    import torch
+    monkeypatch.setenv("FMX_SPLIT", "0")
+    variant, dims = VARIANTS[case]
+    cfg = small_config(variant, dims=dims)
+    dense, comp = run_both(cfg, prec, (37, 22))
+    assert frozen_count(comp["sim"]) > 0
+    if prec == "f16":
+        same = {k: bool(torch.equal(dense[k], comp[k])) for k in ("f", "g", "phi", "q")}
+        errs = {k: rel_err(host(comp[k]), host(dense[k])) for k in ("view", "phi", "q")}
+        say("  bit-identical:", same, " fused compact vs box layout:", {k: f"{v:.1e}" for k, v in errs.items()})
+        assert all(v < 2e-3 for v in errs.values()), errs
+    else:
+        assert_same(dense, comp)
+
+
 @pytest.mark.parametrize("prec", ["f16", "bf16"])
 @pytest.mark.parametrize("case", ["neq_24", "series_96", "wide_256"])
 def test_compact_run_16bit(prec, case):
