@@ -312,7 +312,8 @@ extern template int launch_coupled_split<FMX_BF16>(const CoupledPtrs<FMX_BF16> &
 
 // ------------------------------------------------------------ row-compacted NS storage
 // Workspace (fmx_coupled_bufs::compact_work): the RowSeg table, the scan's chunk sums, a header whose
-// first word is the number of active cells, then the two compact population buffers of 19 planes
+// first word is the number of active cells, the sorption-region masks, then the two compact population
+// buffers of 19 planes
 // each, whose plane stride nc is what the remaining bytes hold (a multiple of 128 cells); with 16-bit
 // storage (the split step) also the box index and link word (8 bytes) and velocity (three values of
 // the compute precision) of each active cell.
@@ -322,7 +323,7 @@ constexpr int kRcChunk = 4096;  // table entries per block of the scan (256 thre
 constexpr int kRcMinSteps = 16;
 
 struct RcLayout {
-    i64 ents, chunks, part_off, hdr_off, planes_off, nc;
+    i64 ents, chunks, part_off, hdr_off, region_off, planes_off, nc;
 };
 // Compact runs take the split step (FMX_SPLIT=0: the fused compact kernel instead); the workspace then
 // holds the per-active-cell record and velocity as well
@@ -341,7 +342,8 @@ static RcLayout rc_layout(const fmx_coupled_desc &d, i64 bytes) {
     l.chunks = (l.ents + kRcChunk - 1) / kRcChunk;
     l.part_off = align256(l.ents * i64(sizeof(RowSeg)));
     l.hdr_off = l.part_off + align256(l.chunks * 4);
-    l.planes_off = l.hdr_off + 256;
+    l.region_off = l.hdr_off + 256;  // sorption-region mask per (row, segment), for the split step's AD half
+    l.planes_off = l.region_off + align256(l.ents * 8);
     const i64 sb = d.prec == FMX_F64 ? 8 : (d.prec == FMX_F32 ? 4 : 2);
     const i64 per_cell = 2 * 19 * sb + (split_step(d.prec) ? split_cell_bytes(d.prec) : 0);
     l.nc = bytes > l.planes_off ? ((bytes - l.planes_off) / per_cell) / 128 * 128 : 0;
@@ -364,18 +366,20 @@ __global__ void __launch_bounds__(256) k_rc_coords(const RowSeg *__restrict__ rs
 
 // one wave per (row, segment): the mask of cells without the frozen bit and its popcount
 __global__ void __launch_bounds__(256) k_rc_masks(const std::uint32_t *__restrict__ links, RowSeg *__restrict__ rs,
-                                                  int nx, int nseg, i64 ents) {
+                                                  unsigned long long *__restrict__ region, int nx, int nseg, i64 ents) {
     const i64 w = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (w >= ents) return;  // wave-uniform
     const unsigned lane = threadIdx.x & 63u;
     const i64 row = w / nseg;
     const int x = int(w - row * nseg) * 64 + int(lane);
-    const bool act = x < nx && !(links[row * nx + x] & FMX_LINK_FROZEN);
-    const unsigned long long m = __ballot(act);
+    const std::uint32_t lk = x < nx ? links[row * nx + x] : FMX_LINK_FROZEN;
+    const unsigned long long m = __ballot(!(lk & FMX_LINK_FROZEN));
+    const unsigned long long r = __ballot((lk & FMX_LINK_REGION) != 0u);
     if (lane == 0) {
         rs[w].lo = unsigned(m);
         rs[w].hi = unsigned(m >> 32);
         rs[w].count = unsigned(__popcll(m));
+        region[w] = r;
     }
 }
 
@@ -483,7 +487,9 @@ static int rc_begin(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
     unsigned long long *hdr = reinterpret_cast<unsigned long long *>(w + l.hdr_off);
     const int nseg = (d.nx + 63) / 64;
     const i64 n = i64(d.nx) * d.ny * d.nz;
-    hipLaunchKernelGGL(k_rc_masks, dim3(unsigned((l.ents + 3) / 4)), dim3(256), 0, s, b.links, rs, d.nx, nseg, l.ents);
+    unsigned long long *region = reinterpret_cast<unsigned long long *>(w + l.region_off);
+    hipLaunchKernelGGL(k_rc_masks, dim3(unsigned((l.ents + 3) / 4)), dim3(256), 0, s, b.links, rs, region, d.nx, nseg,
+                       l.ents);
     if (int rc2 = after_launch("k_rc_masks")) return rc2;
     hipLaunchKernelGGL(k_rc_chunk_sums, dim3(unsigned(l.chunks)), dim3(256), 0, s, rs, l.ents, part);
     if (int rc2 = after_launch("k_rc_chunk_sums")) return rc2;
@@ -506,6 +512,7 @@ static int rc_begin(const fmx_coupled_desc &d, const fmx_coupled_bufs &b, int cu
     if (sa) {  // the split step: box index and velocity of every active cell after the two buffers
         uint2 *coords = reinterpret_cast<uint2 *>(w + l.planes_off + 2 * l.nc * 19 * i64(sizeof(S)));
         sa->rs = rs;
+        sa->region = region;
         sa->nc = l.nc;
         sa->nseg = nseg;
         sa->coords = coords;

@@ -308,6 +308,7 @@ int launch_coupled_rc(const CoupledPtrs<P> &ptr, const CoupledParams<typename St
 // issued for the active cells only.
 struct SplitArgs {
     const RowSeg *rs;
+    const unsigned long long *region;  // sorption-region mask of each (row, segment), bit x % 64
     i64 nc;                  // compact plane stride
     int nseg;
     const uint2 *coords;     // box index and link word of active cell k

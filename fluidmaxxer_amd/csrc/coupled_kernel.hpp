@@ -44,11 +44,13 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     const i64 n = prm.n;
     const Off cell = Off(x) + Off(nx) * (Off(y) + Off(ny) * Off(z));
     const Off cell_b = cell * SB;
-    const std::uint32_t lk = ld_b(p.links, cell * 4u);
+    // the AD half of the split step takes the region and frozen bits from the row tables instead
+    const std::uint32_t lk = ADONLY ? 0u : ld_b(p.links, cell * 4u);
     const bool solid = (OP == FMX_NS_BGK) && (lk & FMX_LINK_SOLID);
     // a frozen solid cell alternates between the two states held in f and f_new and has zero
     // post-collision velocity in both: no NS loads, collision or stores
-    const bool frozen = FZ && (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
+    bool frozen = FZ && (OP == FMX_NS_BGK) && (lk & FMX_LINK_FROZEN);
+    bool in_region = (lk & FMX_LINK_REGION) != 0u;
 
     // Source of the pull along (cx, cy, cz): the cell - c, or the cell itself when that lies outside
     // the box (x is never periodic; y / z wrap when periodic).  The row part is wave-uniform: a scalar
@@ -95,10 +97,16 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     Off uidx = 0;  // ADONLY: compact index of this cell (of the next active one in its row if frozen)
     if constexpr (ADONLY) {
         const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
-        const RowSeg &e = rc.rs[(i64(y) + i64(ny) * z) * rc.nseg + seg];
+        const i64 ent = (i64(y) + i64(ny) * z) * rc.nseg + seg;
+        const RowSeg &e = rc.rs[ent];
         const unsigned lo = __builtin_amdgcn_readfirstlane(e.lo), hi = __builtin_amdgcn_readfirstlane(e.hi);
         const unsigned base = __builtin_amdgcn_readfirstlane(e.base);
+        const unsigned long long rg = rc.region[ent];
+        const unsigned rlo = __builtin_amdgcn_readfirstlane(unsigned(rg)), rhi = __builtin_amdgcn_readfirstlane(unsigned(rg >> 32));
+        const unsigned lane = unsigned(x) & 63u;
         uidx = Off(__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, base)));
+        frozen = (((lane < 32u ? (lo >> lane) : (hi >> (lane - 32u))) & 1u) == 0u);
+        in_region = (((lane < 32u ? (rlo >> lane) : (rhi >> (lane - 32u))) & 1u) != 0u);
     }
     if constexpr (RC) {
         const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
@@ -185,7 +193,7 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
         }
         gi[q] = (prm.ad_inlet == 2 && sx == 0) ? phi_pre * w7<C>(q) : v;
     });
-    const bool sorbs = prm.isotherm && (lk & FMX_LINK_REGION);
+    const bool sorbs = prm.isotherm && in_region;
     // read by every lane (the sorption region is most of a pack's waves); used only where sorbs
     const C phi_lag = ld_b(p.phi, cell * CB);
     const C q_old = ld_b(p.q, cell * CB);

@@ -48,7 +48,8 @@ def test_validate_rejects_bad_descriptors():
 
 def test_compact_workspace_size_and_validation():
     """fmx_coupled_compact_bytes: a row table of 16 bytes per (row, 64-cell segment), 256-byte aligned,
-    the scan's chunk sums and a 256-byte header, then two compact copies of the 19 NS planes of the
+    the scan's chunk sums, a 256-byte header and an 8-byte region mask per (row, segment), then two
+    compact copies of the 19 NS planes of the
     active cells rounded up to 128 cells, and for the split step an 8-byte box index and link word
     and a velocity (three values of the compute precision) per active cell; out-of-range arguments give -1.  A workspace below the
     table's size is refused by validation (no GPU needed)."""
@@ -56,7 +57,8 @@ def test_compact_workspace_size_and_validation():
     n = 130 * 6 * 5
     cfg = CoupledConfig(130, 6, 5, solid=np.zeros(n, np.uint8), region=np.zeros(n, np.uint8))
     ents = 6 * 5 * 3  # rows x ceil(130 / 64)
-    head = (ents * 16 + 255) // 256 * 256 + 256 + 256  # table, chunk sums (one chunk), header
+    head = (ents * 16 + 255) // 256 * 256 + 256 + 256 + (ents * 8 + 255) // 256 * 256  # table, chunk sums
+    # (one chunk), header, sorption-region masks
     for prec, per in ((abi.F32, 2 * 19 * 4 + 8 + 12), (abi.F64, 2 * 19 * 8 + 8 + 24), (abi.F16, 2 * 19 * 2 + 8 + 12),
                       (abi.BF16, 2 * 19 * 2 + 8 + 12)):
         d = cfg.desc(prec)
