@@ -284,7 +284,7 @@ def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", 
     run: per-field errors after the last step (rho, u of the populations as the reference holds them, C,
     q), the outlet breakthrough curve C_out(t) of every step, the breakthrough surface C(x, t) -- the
     plane-averaged concentration of every x-plane every `every` steps, which carries the front before it
-    reaches the outlet -- and the FP64 sums of C and q (the diagnostic log) at the same times.  Errors:
+    reaches the outlet -- and the FP64 sums of C and q (fmx_reduce_mass) at the same times.  Errors:
     max |a - b| / max |b| (and the relative l2 norm for the fields).  A trailing "r" (f16r, bf16r): the
     16-bit storage with its density reference moved to the mean density of the flow every `every` steps
     (CoupledSim.recenter); without it the reference stays at rho0."""
@@ -314,11 +314,11 @@ def precision_study(workload="C3", steps=None, every=1000, precs=("f64", "f32", 
         sim = CoupledSim(cfg, prec=prec)
         curve, surface, mass = [], [], []
         for blk in range(nsteps // every):
-            curve.append(sim.run(every, mass_every=every))
+            curve.append(sim.run(every))  # the library's default layout (row-compacted, split step)
+            mass.append(sim.mass())  # the FP64 sums of the state after the block (before any recentring)
             if recentre:
                 sim.recenter()
             surface.append(sim.phi.view(nz, ny, nx).double().mean(dim=(0, 1)))
-            mass.append(sim.mass_log[0].clone())
             print(f"[precision] {name} step {(blk + 1) * every}/{nsteps} C_out {float(curve[-1][-1]):.3e} "
                   f"front C(x={nx // 4}) {float(surface[-1][nx // 4]):.3e} ({time.perf_counter() - t0:.0f} s)",
                   flush=True)
