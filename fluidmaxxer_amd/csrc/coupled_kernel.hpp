@@ -539,7 +539,8 @@ int launch_coupled_split(const CoupledPtrs<P> &ptr, const CoupledParams<typename
     const char *fw = std::getenv("FMX_WIDE");
     const bool wide = (fw && std::atoi(fw) != 0) || 4 * prm.n * i64(sizeof(typename Storage<P>::S)) > (i64(1) << 32);
     if (sa.nact > 0) {
-        const dim3 g1(unsigned((sa.nact + 255) / 256)), b1(256);
+        // a multiple of 8 workgroups, so that each XCD takes a contiguous range (the remap in the kernel)
+        const dim3 g1(unsigned(((sa.nact + 255) / 256 + 7) / 8 * 8)), b1(256);
         if (wide) hipLaunchKernelGGL((k_ns_split<P, true>), g1, b1, 0, s, ptr, prm, sa);
         else hipLaunchKernelGGL((k_ns_split<P, false>), g1, b1, 0, s, ptr, prm, sa);
         if (int rc = after_launch("k_ns_split")) return rc;
