@@ -369,9 +369,10 @@ __global__ void __launch_bounds__(256, coupled_waves(P))
 
 // The NS work of active cell k (compact index) at (x, y, z) with link word lk, given the compact index
 // of x in each neighbouring row (rsl; this row's is k), that row's active bit at x and whether the row
-// exists (keep-local otherwise): the NS pulls, collision and stores of k_coupled_rc with the boundary
-// writes of its neighbouring-thread form (the x = 0 / 1 and nx-2 / nx-1 cells are never frozen, so
-// they are adjacent in the compact order too), and the cell's velocity stored for the AD half.
+// exists (keep-local otherwise): the NS pulls, collision and stores of k_coupled_rc, with the inlet and
+// outlet values computed by the boundary lanes from lane shuffles (the x = 0 / 1 and nx-2 / nx-1 cells
+// are never frozen, so they are adjacent in the compact order too), and the cell's velocity stored for
+// the AD half.
 template <int P, bool WIDE>
 __device__ __forceinline__ void ns_split_cell(const CoupledPtrs<P> &p, const CoupledParams<typename Storage<P>::C> &prm,
                                               const SplitArgs &sa, i64 k, int x, int y, int z, std::uint32_t lk,
@@ -427,26 +428,49 @@ __device__ __forceinline__ void ns_split_cell(const CoupledPtrs<P> &p, const Cou
 #pragma unroll
         for (int q = 0; q < 19; ++q) p.inlet_raw[i64(q) * np + pc] = fo[q];
     }
-    if (inlet_cell) {
-        if (prm.ns_inlet == 2) {
-            static_for<0, 19>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                st_plane_nt(plane(p.fn, q, sa.nc), ncell_b, encode<P>(inlet_eq<q>(prm), C(Q19::inv_w[q]), prm.f_ref));
-            });
+    // Boundary lanes compute their own values from their neighbour's, handed over by lane shuffles:
+    // x = 1 is the next compact cell after x = 0, x = nx-2 the one before x = nx-1, so the neighbour is
+    // the adjacent lane unless the pair straddles a wave boundary (lane 63 / lane 0 of the next wave).
+    // Those pairs keep the neighbouring-thread write below; every other wave stores each population
+    // plane with one full-wave store.
+    const unsigned lane = unsigned(k) & 63u;
+    const bool in_edge = prm.ns_inlet == 1 && inlet_cell && lane == 63u;
+    const bool out_edge = outlet_cell && lane == 0u;
+    if (__any(inlet_cell || outlet_cell)) {
+        C fp[19], rp = C(0), up_x = C(0), up_y = C(0), up_z = C(0), um_x = C(0), um_y = C(0), um_z = C(0);
+        if (prm.ns_inlet == 1) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) fp[q] = __shfl_down(fo[q], 1, 64);
+            rp = __shfl_down(r2, 1, 64);
+            up_x = __shfl_down(ux2, 1, 64);
+            up_y = __shfl_down(uy2, 1, 64);
+            up_z = __shfl_down(uz2, 1, 64);
         }
-    } else if (!outlet_cell) {
+        if (prm.ns_outlet) {
+            um_x = __shfl_up(ux2, 1, 64);
+            um_y = __shfl_up(uy2, 1, 64);
+            um_z = __shfl_up(uz2, 1, 64);
+        }
+        if (!(in_edge || out_edge)) static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C v = fo[q];
+            if (inlet_cell) v = (prm.ns_inlet == 2) ? inlet_eq<q>(prm) : inlet_neq<q>(fp[q], rp, up_x, up_y, up_z, prm);
+            else if (outlet_cell) v = outlet_pressure<q>(um_x, um_y, um_z, prm);
+            st_plane_nt(plane(p.fn, q, sa.nc), ncell_b, encode<P>(v, C(Q19::inv_w[q]), prm.f_ref));
+        });
+    } else {
 #pragma unroll
         for (int q = 0; q < 19; ++q)
             st_plane_nt(plane(p.fn, q, sa.nc), ncell_b, encode<P>(fo[q], C(Q19::inv_w[q]), prm.f_ref));
     }
-    if (prm.ns_inlet == 1 && x == 1) {
+    if (prm.ns_inlet == 1 && x == 1 && lane == 0u) {
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             st_b(plane(p.fn, q, sa.nc), ncell_b - SB,
                  encode<P>(inlet_neq<q>(fo[q], r2, ux2, uy2, uz2, prm), C(Q19::inv_w[q]), prm.f_ref));
         });
     }
-    if (prm.ns_outlet && x == nx - 2) {
+    if (prm.ns_outlet && x == nx - 2 && lane == 63u) {
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             st_b(plane(p.fn, q, sa.nc), ncell_b + SB,

@@ -34,6 +34,16 @@ def frozen_count(sim):
     return int((sim.links & abi.LINK_FROZEN).ne(0).sum().item())
 
 
+def wave_straddling_pairs(cfg, sim):
+    """Rows whose inlet pair (x = 0, 1) or outlet pair (x = nx-2, nx-1) falls in two waves of the split
+    NS kernel (the first cell of the pair in lane 63 of the compact order): the boundary lanes of such
+    pairs cannot take their neighbour's values by a lane shuffle, so the neighbour writes them."""
+    frozen = (sim.links.cpu().numpy().astype(np.int64) & abi.LINK_FROZEN) != 0
+    k = np.cumsum(~frozen) - 1
+    x = np.arange(cfg.n) % cfg.nx
+    return int(((x == 0) & (k % 64 == 63)).sum()), int(((x == cfg.nx - 2) & (k % 64 == 63)).sum())
+
+
 def run_both(cfg, prec, runs, seed=5, mass_every=0):
     out = []
     for compact in (False, True):
@@ -80,8 +90,12 @@ def test_compact_run_bit_identical(prec, case):
     cfg = small_config(variant, dims=dims)
     dense, comp = run_both(cfg, prec, (37, 1, 22))
     nfz = frozen_count(comp["sim"])
-    say(f"  frozen cells {nfz} of {cfg.n}; workspace {comp['sim']._compact_ws.numel()} B")
+    pairs = wave_straddling_pairs(cfg, comp["sim"])
+    say(f"  frozen cells {nfz} of {cfg.n}; workspace {comp['sim']._compact_ws.numel()} B; "
+        f"inlet / outlet pairs across two waves {pairs}")
     assert nfz > 0 and frozen_count(dense["sim"]) == 0
+    if case == "neq_24":  # both kinds of boundary pair split between two waves are exercised
+        assert pairs[0] > 0 and pairs[1] > 0
     assert_same(dense, comp)
 
 
