@@ -313,7 +313,7 @@ struct SplitArgs {
     int nseg;
     const uint2 *coords;     // box index and link word of active cell k
     i64 nact;
-    void *u;                 // ux, uy, uz of active cell k at k, nc + k, 2 nc + k (compute precision)
+    void *u;                 // ux, uy, uz of active cell k at 3k, 3k + 1, 3k + 2 (compute precision)
 };
 template <int P>
 int launch_coupled_split(const CoupledPtrs<P> &ptr, const CoupledParams<typename Storage<P>::C> &prm, i64 step,

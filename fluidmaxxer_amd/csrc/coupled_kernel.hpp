@@ -202,11 +202,12 @@ __device__ __forceinline__ void coupled_body(const CoupledPtrs<P> &p, const Coup
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
     if constexpr (ADONLY) {
         if (prm.coupling) {  // every lane loads (its index is in the box of active cells); frozen: zero
-            const C *u = static_cast<const C *>(rc.u);
-            const Off nub = Off(rc.nc) * CB;
-            ux2 = frozen ? C(0) : ld_b(u, uidx * CB);
-            uy2 = frozen ? C(0) : ld_b(u, nub + uidx * CB);
-            uz2 = frozen ? C(0) : ld_b(u, nub + nub + uidx * CB);
+            // (ux, uy, uz) of a cell side by side: one wide load
+            C uv[3];
+            __builtin_memcpy(uv, static_cast<const C *>(rc.u) + std::size_t(uidx) * 3u, sizeof(uv));
+            ux2 = frozen ? C(0) : uv[0];
+            uy2 = frozen ? C(0) : uv[1];
+            uz2 = frozen ? C(0) : uv[2];
         }
     } else if (RC || !frozen) {
         const C fld = p.field ? ld_b(p.field, cell * CB) : ((OP == FMX_NS_POROUS) ? C(1) : C(0));
@@ -417,10 +418,10 @@ __device__ __forceinline__ void ns_split_cell(const CoupledPtrs<P> &p, const Cou
     C r2 = C(0), ux2 = C(0), uy2 = C(0), uz2 = C(0);
     ns_collide_cell<FMX_NS_BGK, IsHalf<P>::value>(fo, solid, C(0), prm);
     post_moments<IsHalf<P>::value>(fo, r2, ux2, uy2, uz2);
-    C *u = static_cast<C *>(sa.u);
-    u[k] = ux2;
-    u[sa.nc + k] = uy2;
-    u[2 * sa.nc + k] = uz2;
+    C *u = static_cast<C *>(sa.u) + 3 * k;  // one wide store per lane
+    u[0] = ux2;
+    u[1] = uy2;
+    u[2] = uz2;
     const bool inlet_cell = prm.ns_inlet && x == 0;
     const bool outlet_cell = prm.ns_outlet && x == nx - 1;
     if (inlet_cell && p.inlet_raw) {
