@@ -30,13 +30,16 @@ def test_bench_line_contract():
     assert abs(d["value"] - cells / (d["ms_per_step"] * 1e3)) <= 1e-3 * d["value"]
     rf = d["roofline"]
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
-    assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3 and 0 < rf["frac"] < 1
+    # on 225 B per cell update the fraction can pass 1 (the row-compacted layout does not move the frozen
+    # solid cells' NS populations); on the bytes moved it cannot
+    assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3 and 0 < rf["frac"] < 1.6
     # FP32 runs in row-compacted NS planes by default: the counter summary of that kernel, and the same
     # time over the bytes the compact run moves
     assert rf["traffic"] > 0 and rf["traffic_source"].startswith("profiles/traffic_rc_c2_f32.json")
     assert d["config"]["ns_layout"].startswith("row-compacted") and rf["kernel"].startswith("k_ns_split + k_ad_split")
     cl = rf["compact_layout"]
     assert 0.3 < cl["active_fraction"] < 0.5 and cl["bytes_per_cell_update"] < rf["bytes_per_cell_update"]
+    assert 0 < cl["frac"] < 1
     assert abs(cl["frac"] / rf["frac"] - cl["bytes_per_cell_update"] / rf["bytes_per_cell_update"]) < 1e-3
     cb = d["cpu_baseline"]
     assert cb["cores"] == 1 and cb["kind"] in ("port", "reference") and cb["value"] > 0 and cb["sample"]
